@@ -1,0 +1,549 @@
+/*
+ * hk_oracle.c -- FP64 host oracle for the HELIOS-K line-by-line path.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it as the checker / CPU restatement.
+ * The product path never links or loads it.
+ *
+ * What each part restates and what pins it:
+ *
+ *  hko_erfcx, hko_w916, hko_wcf, hko_faddeeva
+ *      Faddeeva function.  hko_w916 is Zaghloul & Ali, "Algorithm 916"
+ *      (ACM TOMS 38(2), 2011), eqs. for Re w (as in voigt.h:165-216 and
+ *      voigt.h:273-296 of the reference) plus the matching Im w series.
+ *      hko_wcf is the Laplace continued fraction
+ *      w(z) = (i/sqrt(pi)) / (z - (1/2)/(z - 1/(z - (3/2)/(z - ...)))).
+ *      Pinned by 50-digit mpmath golden vectors (tests/golden) and by
+ *      scipy.special.wofz, plus the closed forms w(iy) = erfcx(y),
+ *      Re w(x,0) = exp(-x^2), and integral(Re w dx) = sqrt(pi).
+ *  hko_line_params
+ *      host.h:641-644 (readFile: vy, 1/alpha_D, S/Q) followed by
+ *      voigt.h:338-356 (S2_kernel) and the cut of voigt.h:442-450
+ *      (S3_kernel) expressed as a half-width in cm^-1.  Pinned by closed
+ *      forms (S(T0) = S_file/Q*Ab, alpha_D ~ nu sqrt(T), alpha_L ~ P
+ *      (T0/T)^n) checked in tests/test_oracle.py.
+ *  hko_window
+ *      The cut test |x_j - nu_i| < cut (voigt.h:1223 with vcut2 of
+ *      voigt.h:442-450) evaluated in FP64 on x_j = fma(j, dnu, numin)
+ *      (setX_kernel, voigt.h:751-759).  Pinned by brute force.
+ *  hko_voigt_points
+ *      K_j = kmin + sum_i S_i Re w(x_ij, y_i)/sqrt(pi): the sum the
+ *      Line6fA/B/C kernels accumulate (voigt.h:1177-1281, 1437-1515,
+ *      1616-1706) onto InitialK's kmin (voigt.h:713-722).  Pinned by
+ *      linearity, translation, symmetry and the normalisation
+ *      sum_j K_j dnu -> sum_i S_i for an uncut line.
+ *  hko_voigt_grad
+ *      Analytic gradient of sum_j gK_j K_j; pinned by central finite
+ *      differences of hko_voigt_points.
+ *  hko_bin_segments, hko_sort_segments
+ *      binKey_kernel (voigt.h:779-806) + the two thrust sorts of
+ *      heliosk.cu:2049-2050.  Pinned by plots/p004 and plots/p009.
+ *  hko_qr, hko_ktable, hko_reconstruct
+ *      resample.h:14-37 (Vandermonde), 73-227 (QR), 243-353 (least squares,
+ *      NC-1 reflections), 362-388 (findCut), 398-437 (rescale),
+ *      472-481 (lnK), 497-524 (expfx).  Pinned by plots/p004 and plots/p005
+ *      (Out_r_cbin.dat, Out_r_bin.dat).
+ *
+ * Floating-point contraction: the reference was compiled by nvcc, whose
+ * default --fmad=true turns a*b+c into one fused multiply-add.  Where the
+ * reference writes such an expression and the rounding matters (grid
+ * positions, rescale interpolation) this file writes fma() explicitly.
+ */
+#include "hk_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define HKO_SQRTPI  1.77245385090551602730
+#define HKO_ISQRTPI 0.56418958354775628695
+
+/* ------------------------------------------------------------------ */
+/* erfcx(y) = exp(y^2) erfc(y), y >= 0                                 */
+/* ------------------------------------------------------------------ */
+double hko_erfcx(double y)
+{
+    if (y < 0.0) return 2.0 * exp(y * y) - hko_erfcx(-y);
+    if (y < 5.0) return exp(y * y) * erfc(y);
+    /* Laplace continued fraction, evaluated backward. */
+    double t = y;
+    for (int k = 60; k >= 1; --k) t = y + 0.5 * k / t;
+    return HKO_ISQRTPI / t;
+}
+
+static double sinc(double t) { return t == 0.0 ? 1.0 : sin(t) / t; }
+
+/* ------------------------------------------------------------------ */
+/* Algorithm 916 (Zaghloul & Ali 2011), FP64, real and imaginary part. */
+/* Series truncation: all terms below exp(-45) relative are dropped.   */
+/* ------------------------------------------------------------------ */
+void hko_w916(double x, double y, double tol, double *re, double *im)
+{
+    double sgn = x < 0.0 ? -1.0 : 1.0;
+    x = fabs(x);
+    if (x == 0.0) { *re = hko_erfcx(y); *im = 0.0; return; }
+    double a = M_PI / sqrt(-log(tol / 2.0));
+    double yy = y * y, xx = x * x;
+    double ex2 = exp(-xx);
+    int N1 = (int)ceil(6.8 / a) + 1;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
+    for (int n = 1; n <= N1; ++n) {
+        double an = a * n, den = an * an + yy;
+        double e1 = exp(-(an * an + xx));
+        double e2 = exp(-(an + x) * (an + x));
+        s1 += e1 / den;
+        s2 += e2 / den;
+        s5 += an * e2 / den;
+    }
+    long n0 = (long)ceil(x / a);
+    long nlo = n0 - N1; if (nlo < 1) nlo = 1;
+    long nhi = n0 + N1;
+    for (long n = nlo; n <= nhi; ++n) {
+        double an = a * (double)n, den = an * an + yy;
+        double e3 = exp(-(an - x) * (an - x));
+        s3 += e3 / den;
+        s4 += an * e3 / den;
+    }
+    double xy = x * y, c2 = cos(2.0 * xy), s2xy = sin(2.0 * xy);
+    double sxy = sin(xy);
+    double ca = 2.0 * a / M_PI;
+    double ex = hko_erfcx(y);
+    double r, i;
+    if (y == 0.0) {
+        r = ex2;
+    } else {
+        r = ex2 * ex * c2 + ca * (sxy * x * sinc(xy) * ex2
+                                  + y * (-c2 * s1 + 0.5 * (s2 + s3)));
+    }
+    i = -ex2 * ex * s2xy + ca * (x * ex2 * sinc(2.0 * xy) + y * s2xy * s1
+                                 + 0.5 * s4 - 0.5 * s5);
+    *re = r;
+    *im = sgn * i;
+}
+
+/* ------------------------------------------------------------------ */
+/* Laplace continued fraction, backward evaluation in real arithmetic. */
+/* t_N = z; t_k = z - (k/2)/t_{k+1}; w = i / (sqrt(pi) t_1).           */
+/* ------------------------------------------------------------------ */
+void hko_wcf(double x, double y, int depth, double *re, double *im)
+{
+    double tr = x, ti = y;
+    for (int k = depth; k >= 1; --k) {
+        double m = 0.5 * k / (tr * tr + ti * ti);
+        tr = x - m * tr;
+        ti = y + m * ti;
+    }
+    double q = HKO_ISQRTPI / (tr * tr + ti * ti);
+    *re = ti * q;
+    *im = tr * q;
+}
+
+void hko_faddeeva(double x, double y, double *re, double *im)
+{
+    double r2 = x * x + y * y;
+    if (r2 < 100.0) { hko_w916(x, y, 1.0e-17, re, im); return; }
+    int depth = r2 >= 1.0e4 ? 8 : (r2 >= 1.0e3 ? 10 : (r2 >= 400.0 ? 12 : 20));
+    hko_wcf(x, y, depth, re, im);
+    if (y == 0.0) *re = exp(-x * x);
+}
+
+void hko_faddeeva_batch(int64_t n, const double *x, const double *y,
+                        double *re, double *im)
+{
+    #pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) hko_faddeeva(x[k], y[k], &re[k], &im[k]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Line parameters                                                     */
+/* ------------------------------------------------------------------ */
+void hko_line_params(int64_t nl,
+                     const double *nu0, const double *S0, const double *EL,
+                     const double *A, const double *delta, const double *vy0,
+                     const double *nexp, const int32_t *iso,
+                     const double *iso_mass, const double *iso_ab,
+                     const double *iso_sscale, const double *iso_Q,
+                     double T, double P, int cutMode, double cut,
+                     double *nu_o, double *S_o, double *ia_o, double *y_o,
+                     double *hw_o)
+{
+    const double c = HKO_H * HKO_C / HKO_KB;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nl; ++i) {
+        int k = iso ? iso[i] : 0;
+        /* readFile, host.h:641-644 */
+        double ialphaD_h = HKO_C * sqrt(iso_mass[k] / (2.0 * HKO_KB * T));
+        double Sh = S0[i] / iso_Q[k] * iso_ab[k] * iso_sscale[k];
+        /* S2_kernel, voigt.h:338-356 */
+        double nu = fma(delta[i], P, nu0[i]);
+        double ialphaD = ialphaD_h / nu;
+        double E = EL[i];
+        double S = Sh * (exp(-E * c / T + E * c / HKO_T0) * (1.0 - exp(-c * nu / T))
+                         / (1.0 - exp(-c * nu / HKO_T0)) * ialphaD);
+        double alphaL = fma(vy0[i], (P / HKO_POATM) * pow(HKO_T0 / T, nexp[i]),
+                            A[i] / (4.0 * M_PI * HKO_C));
+        double y = alphaL * ialphaD;
+        double hw;
+        if (cutMode == 1) hw = cut * alphaL;
+        else if (cutMode == 2) hw = cut / ialphaD;
+        else hw = cut;
+        if (nu == 0.0) { S = 0.0; ialphaD = 0.0; y = 0.0; hw = -1.0; }
+        nu_o[i] = nu; S_o[i] = S; ia_o[i] = ialphaD; y_o[i] = y; hw_o[i] = hw;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Window: j in [jlo, jhi)  <=>  |fma(j,dnu,numin) - nu| < hw           */
+/* ------------------------------------------------------------------ */
+static int64_t clamp_index(double e, int64_t Nx)
+{
+    if (!(e > 0.0)) return 0;             /* also NaN */
+    if (e >= (double)Nx) return Nx;
+    return (int64_t)floor(e);
+}
+
+void hko_window(double nu, double hw, double numin, double dnu, int64_t Nx,
+                int64_t *jlo, int64_t *jhi)
+{
+    if (!(hw > 0.0)) { *jlo = 0; *jhi = 0; return; }
+    int64_t lo = clamp_index((nu - hw - numin) / dnu, Nx);
+    while (lo > 0 && fma((double)(lo - 1), dnu, numin) - nu > -hw) --lo;
+    while (lo < Nx && !(fma((double)lo, dnu, numin) - nu > -hw)) ++lo;
+    int64_t hi = clamp_index((nu + hw - numin) / dnu, Nx);
+    while (hi > 0 && !(fma((double)(hi - 1), dnu, numin) - nu < hw)) --hi;
+    while (hi < Nx && fma((double)hi, dnu, numin) - nu < hw) ++hi;
+    if (hi < lo) hi = lo;
+    *jlo = lo; *jhi = hi;
+}
+
+/* ------------------------------------------------------------------ */
+/* Voigt sum (gather over points, Neumaier-compensated)                */
+/* ------------------------------------------------------------------ */
+static int64_t lower_bound(const double *v, int64_t n, double key)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) { int64_t m = (lo + hi) / 2; if (v[m] < key) lo = m + 1; else hi = m; }
+    return lo;
+}
+
+void hko_voigt_points(int64_t nl, const double *nu, const double *S,
+                      const double *ialphaD, const double *y,
+                      const double *hw, double R,
+                      double numin, double dnu, int64_t Nx, double kmin,
+                      int64_t npts, const int64_t *jidx, double *K,
+                      int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t p = 0; p < npts; ++p) {
+        int64_t j = jidx ? jidx[p] : p;
+        double xj = fma((double)j, dnu, numin);
+        int64_t i0 = lower_bound(nu, nl, xj - R);
+        int64_t i1 = lower_bound(nu, nl, xj + R);
+        if (i1 < nl) ++i1;
+        double sum = kmin, comp = 0.0;
+        for (int64_t i = i0; i < i1; ++i) {
+            double d = xj - nu[i];
+            if (!(d < hw[i] && d > -hw[i])) continue;
+            double re, im;
+            hko_faddeeva(d * ialphaD[i], y[i], &re, &im);
+            double t = S[i] * re * HKO_ISQRTPI;
+            double s2 = sum + t;
+            if (fabs(sum) >= fabs(t)) comp += (sum - s2) + t;
+            else comp += (t - s2) + sum;
+            sum = s2;
+        }
+        K[p] = sum + comp;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Analytic gradient of L = sum_j gK_j K_j                             */
+/* with respect to T, P (gTP[0], gTP[1]) and per-line S0, vy0.         */
+/* The partition function Q is held fixed (constant ratio).            */
+/* ------------------------------------------------------------------ */
+void hko_voigt_grad(int64_t nl,
+                    const double *nu0, const double *S0, const double *EL,
+                    const double *A, const double *delta, const double *vy0,
+                    const double *nexp, const int32_t *iso,
+                    const double *iso_mass, const double *iso_ab,
+                    const double *iso_sscale, const double *iso_Q,
+                    double T, double P, int cutMode, double cut,
+                    double numin, double dnu, int64_t Nx, const double *gK,
+                    double *gS0, double *gvy0, double *gTP, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const double c = HKO_H * HKO_C / HKO_KB;
+    long double gT = 0.0L, gP = 0.0L;
+    #pragma omp parallel for schedule(dynamic, 256) reduction(+:gT, gP)
+    for (int64_t i = 0; i < nl; ++i) {
+        int k = iso ? iso[i] : 0;
+        double ialphaD_h = HKO_C * sqrt(iso_mass[k] / (2.0 * HKO_KB * T));
+        double F0 = 1.0 / iso_Q[k] * iso_ab[k] * iso_sscale[k];
+        double nu = fma(delta[i], P, nu0[i]);
+        if (nu == 0.0) { gS0[i] = 0.0; gvy0[i] = 0.0; continue; }
+        double ia = ialphaD_h / nu;
+        double E = EL[i];
+        double B = exp(-E * c / T + E * c / HKO_T0);
+        double eT = exp(-c * nu / T), e0 = exp(-c * nu / HKO_T0);
+        double Rr = (1.0 - eT) / (1.0 - e0);
+        double F = F0 * B * Rr * ia;            /* S = S0 * F */
+        double S = S0[i] * F;
+        double pw = pow(HKO_T0 / T, nexp[i]);
+        double aLp = vy0[i] * P * pw;           /* pressure part of alpha_L */
+        double aL = aLp + A[i] / (4.0 * M_PI * HKO_C);
+        double y = aL * ia;
+        double hw = cutMode == 1 ? cut * aL : (cutMode == 2 ? cut / ia : cut);
+        /* log-derivatives */
+        double dlnR_dT = -eT * (c * nu / (T * T)) / (1.0 - eT);
+        double dlnR_dnu = eT * (c / T) / (1.0 - eT) - e0 * (c / HKO_T0) / (1.0 - e0);
+        double dlnS_dT = c * E / (T * T) + dlnR_dT - 0.5 / T;
+        double dlnS_dP = delta[i] * (dlnR_dnu - 1.0 / nu);
+        double dia_dT = -0.5 * ia / T, dia_dP = -ia * delta[i] / nu;
+        double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * pw;
+        double dy_dT = daL_dT * ia + aL * dia_dT;
+        double dy_dP = daL_dP * ia + aL * dia_dP;
+        double dy_dvy = P * pw * ia;
+        int64_t jlo, jhi;
+        hko_window(nu, hw, numin, dnu, Nx, &jlo, &jhi);
+        double sre = 0.0, sdy = 0.0, sT = 0.0, sP = 0.0;
+        for (int64_t j = jlo; j < jhi; ++j) {
+            double g = gK[j];
+            if (g == 0.0) continue;
+            double d = fma((double)j, dnu, numin) - nu;
+            double x = d * ia;
+            double re, im;
+            hko_faddeeva(x, y, &re, &im);
+            double dRe_dx = -2.0 * (x * re - y * im);
+            double dRe_dy = 2.0 * (x * im + y * re) - 2.0 * HKO_ISQRTPI;
+            double dx_dT = d * dia_dT;
+            double dx_dP = -delta[i] * ia + d * dia_dP;
+            sre += g * re;
+            sdy += g * dRe_dy;
+            sT += g * (re * dlnS_dT + dRe_dx * dx_dT + dRe_dy * dy_dT);
+            sP += g * (re * dlnS_dP + dRe_dx * dx_dP + dRe_dy * dy_dP);
+        }
+        gS0[i] = F * sre * HKO_ISQRTPI;
+        gvy0[i] = S * sdy * dy_dvy * HKO_ISQRTPI;
+        gT += (long double)(S * sT * HKO_ISQRTPI);
+        gP += (long double)(S * sP * HKO_ISQRTPI);
+    }
+    gTP[0] = (double)gT;
+    gTP[1] = (double)gP;
+}
+
+/* ------------------------------------------------------------------ */
+/* Bins (binKey_kernel semantics) and per-bin sort                     */
+/* ------------------------------------------------------------------ */
+static int bin_key(double xj, const double *bnd, int nbins, double numax)
+{
+    int key = 0;
+    for (int i = 0; i < nbins; ++i)
+        if (bnd[i] <= xj && xj < bnd[i + 1]) { key = i; break; }
+    if (xj >= numax) key = nbins;
+    return key;
+}
+
+/* seg[b] = first grid index whose bin key is >= b, b = 0..nbins+1 */
+void hko_bin_segments(double numin, double numax, double dnu, int64_t Nx,
+                      int nbins, int64_t *seg)
+{
+    double *bnd = (double *)malloc((nbins + 1) * sizeof(double));
+    for (int i = 0; i < nbins; ++i)
+        bnd[i] = numin + i * (numax - numin) / ((double)nbins);   /* heliosk.cu:438-441 */
+    bnd[nbins] = numax;
+    for (int b = 0; b <= nbins + 1; ++b) {
+        int64_t lo = 0, hi = Nx;      /* keys are nondecreasing in j */
+        while (lo < hi) {
+            int64_t m = (lo + hi) / 2;
+            if (bin_key(fma((double)m, dnu, numin), bnd, nbins, numax) < b) lo = m + 1;
+            else hi = m;
+        }
+        seg[b] = lo;
+    }
+    free(bnd);
+}
+
+static int cmp_double(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+void hko_sort_segments(double *K, int nseg, const int64_t *seg)
+{
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nseg; ++b)
+        qsort(K + seg[b], (size_t)(seg[b + 1] - seg[b]), sizeof(double), cmp_double);
+}
+
+/* ------------------------------------------------------------------ */
+/* Vandermonde + Householder QR (resample.h:14-37, 73-227)             */
+/* V is column-major Nxb x nC.                                         */
+/* ------------------------------------------------------------------ */
+void hko_qr(int NL, int NC, double *V, double *C, double *D)
+{
+    double nl = (double)NL;
+    for (int id = 0; id < NL; ++id) {
+        double x = id * 2.0 / (nl - 1.0) - 1.0;
+        V[id] = 1.0;
+        if (NC > 1) V[id + (size_t)NL] = x;
+        double t = x, t0 = 1.0;
+        for (int i = 2; i < NC; ++i) {
+            double tt = t;
+            t = fma(2.0 * x, t, -t0);
+            t0 = tt;
+            V[id + (size_t)NL * i] = t;
+        }
+    }
+    for (int i = 0; i < NC; ++i) {
+        double *vi = V + (size_t)NL * i;
+        double scale = 0.0;
+        for (int k = i; k < NL; ++k) scale = fmax(scale, fabs(vi[k]));
+        double ss = 0.0;
+        for (int k = i; k < NL; ++k) { vi[k] /= scale; ss += vi[k] * vi[k]; }
+        double s = sqrt(ss), v = vi[i];
+        double sigma = v < 0.0 ? -s : s;
+        vi[i] += sigma;
+        C[i] = sigma * (v + sigma);
+        D[i] = -scale * sigma;
+        for (int j = i + 1; j < NC; ++j) {
+            double *vj = V + (size_t)NL * j;
+            double dot = 0.0;
+            for (int k = i; k < NL; ++k) dot += vi[k] * vj[k];
+            double tq = dot / C[i];
+            for (int k = i; k < NL; ++k) vj[k] -= tq * vi[k];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* k-table: findCut, rescale(+1), copyK2, lnK, leastSquare             */
+/* K (length Nx = nbins*Nxb) is the per-bin sorted cross-section; it is */
+/* overwritten with ln K (as the reference does with K_d).             */
+/* coeff[b*nC + c], Nxmin[b].                                          */
+/* ------------------------------------------------------------------ */
+void hko_ktable(double *K, int64_t Nx, int Nxb, int nbins, int nC,
+                double kmin, double unitScale, double *coeff, int *Nxmin)
+{
+    /* findCut_kernel, resample.h:362-388 (rule order = statement order) */
+    for (int b = 0; b < nbins; ++b) Nxmin[b] = 0;
+    int *r1 = (int *)malloc(nbins * sizeof(int));
+    int *r2 = (int *)malloc(nbins * sizeof(int));
+    int *r3 = (int *)malloc(nbins * sizeof(int));
+    for (int b = 0; b < nbins; ++b) { r1[b] = -1; r2[b] = -1; r3[b] = -1; }
+    for (int64_t id = 0; id < Nx - 1; ++id) {
+        int ib = (int)(id / Nxb);
+        if (ib >= nbins) continue;
+        double k0 = K[id], k1 = K[id + 1];
+        if (k0 <= kmin && k1 > kmin) r1[ib] = (int)(id - (int64_t)ib * Nxb + 1);
+        if (k0 <= kmin && id % Nxb == Nxb - 1) r2[ib] = Nxb;
+        if (k0 <= kmin && id == Nx - 3) r3[ib] = Nxb;
+    }
+    for (int b = 0; b < nbins; ++b) {
+        if (r1[b] >= 0) Nxmin[b] = r1[b];
+        if (r2[b] >= 0) Nxmin[b] = r2[b];
+        if (r3[b] >= 0) Nxmin[b] = r3[b];
+    }
+    free(r1); free(r2); free(r3);
+
+    /* rescale_kernel f=+1 and copyK2_kernel, resample.h:398-466 */
+    double *K2 = (double *)malloc((size_t)Nxb * sizeof(double));
+    for (int b = 0; b < nbins; ++b) {
+        int n = Nxmin[b];
+        if (n <= 0) continue;
+        double *Kb = K + (size_t)b * Nxb;
+        double s = 1.0 - n / ((double)(Nxb - 1));
+        for (int k = 0; k < Nxb; ++k) {
+            double ii = fma(s, (double)k, (double)n);
+            if (ii >= Nxb - 1) ii = 0.999999 * (Nxb - 1);
+            int il = (int)(ii / ((double)Nxb) * Nxb);
+            double Kl = Kb[il], Kr = Kb[il + 1];
+            K2[k] = fma(Kr - Kl, ii - il, Kl);
+        }
+        memcpy(Kb, K2, (size_t)Nxb * sizeof(double));
+    }
+    free(K2);
+
+    /* lnK_kernel, resample.h:472-481 */
+    for (int64_t id = 0; id < Nx; ++id) K[id] = log(K[id] * unitScale);
+
+    /* leastSquare_kernel, resample.h:243-353 */
+    double *V = (double *)malloc((size_t)Nxb * nC * sizeof(double));
+    double *C = (double *)malloc(nC * sizeof(double));
+    double *D = (double *)malloc(nC * sizeof(double));
+    hko_qr(Nxb, nC, V, C, D);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nbins; ++b) {
+        double *bb = K + (size_t)b * Nxb;
+        for (int j = 0; j < nC - 1; ++j) {
+            const double *vj = V + (size_t)Nxb * j;
+            double dot = 0.0;
+            for (int k = j; k < Nxb; ++k) dot += vj[k] * bb[k];
+            double t = dot / C[j];
+            for (int k = j; k < Nxb; ++k) bb[k] -= t * vj[k];
+        }
+        bb[nC - 1] /= D[nC - 1];
+        for (int i = nC - 2; i >= 0; --i) {
+            double a = 0.0;
+            for (int j = i + 1; j < nC; ++j) a += V[i + (size_t)Nxb * j] * bb[j];
+            bb[i] = (bb[i] - a) / D[i];
+        }
+        for (int c = 0; c < nC; ++c) coeff[(size_t)b * nC + c] = bb[c];
+    }
+    free(V); free(C); free(D);
+}
+
+/* ------------------------------------------------------------------ */
+/* Reconstruction K(y): expfx_kernel + rescale(f=-1) (resample.h:497-524, */
+/* 398-437); Ky has nbins*Nxb entries.                                 */
+/* ------------------------------------------------------------------ */
+void hko_reconstruct(const double *coeff, const int *Nxmin, int Nxb,
+                     int nbins, int nC, double kmin, double unitScale,
+                     double *Ky)
+{
+    double *F = (double *)malloc((size_t)Nxb * sizeof(double));
+    for (int b = 0; b < nbins; ++b) {
+        const double *cb = coeff + (size_t)b * nC;
+        for (int k = 0; k < Nxb; ++k) {
+            double x = -1.0 + k * 2.0 / ((double)Nxb);
+            double d1 = 0.0, d2 = 0.0;
+            for (int i = nC - 1; i >= 1; --i) {
+                double t = d1;
+                d1 = fma(2.0 * x, d1, -d2) + cb[i];
+                d2 = t;
+            }
+            double f = fma(x, d1, -d2) + 0.5 * cb[0] + 0.5 * cb[0];
+            F[k] = exp(f) / unitScale;
+        }
+        double *out = Ky + (size_t)b * Nxb;
+        int n = Nxmin[b];
+        if (n > 0) {
+            for (int k = 0; k < Nxb; ++k) {
+                double Ki = kmin;
+                if (k >= n) {
+                    double ii = (k - n) / ((double)(Nxb - n - 1)) * (Nxb - 1);
+                    if (ii >= Nxb - 1) ii = 0.999999 * (Nxb - 1);
+                    /* Nxmin == Nxb-1 gives 0/0; the device conversion of NaN is 0 */
+                    int il = ii == ii ? (int)(ii / ((double)Nxb) * Nxb) : 0;
+                    double Kl = F[il], Kr = F[il + 1];
+                    Ki = fma(Kr - Kl, ii - il, Kl);
+                }
+                out[k] = Ki;
+            }
+        } else {
+            memcpy(out, F, (size_t)Nxb * sizeof(double));
+        }
+    }
+    free(F);
+}

@@ -1,0 +1,186 @@
+"""ctypes binding of the FP64 host oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke()
+(as the checker) and by the cpu_baseline leg of bench.py.  The product path
+(the HIP library and the heliosk driver) never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_INT = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_d, c_i64, c_int = ctypes.c_double, ctypes.c_int64, ctypes.c_int
+        L.hko_erfcx.restype = c_d
+        L.hko_erfcx.argtypes = [c_d]
+        L.hko_faddeeva_batch.argtypes = [c_i64, _D, _D, _D, _D]
+        L.hko_w916.argtypes = [c_d, c_d, c_d, _D, _D]
+        L.hko_wcf.argtypes = [c_d, c_d, c_int, _D, _D]
+        L.hko_line_params.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
+                                      + [c_d, c_d, c_int, c_d] + [_D] * 5)
+        L.hko_window.argtypes = [c_d, c_d, c_d, c_d, c_i64, _I64, _I64]
+        L.hko_voigt_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d,
+                                                            c_i64, _I64, _D, c_int])
+        L.hko_voigt_grad.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
+                                     + [c_d, c_d, c_int, c_d, c_d, c_d, c_i64, _D,
+                                        _D, _D, _D, c_int])
+        L.hko_bin_segments.argtypes = [c_d, c_d, c_d, c_i64, c_int, _I64]
+        L.hko_sort_segments.argtypes = [_D, c_int, _I64]
+        L.hko_qr.argtypes = [c_int, c_int, _D, _D, _D]
+        L.hko_ktable.argtypes = [_D, c_i64, c_int, c_int, c_int, c_d, c_d, _D, _INT]
+        L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def erfcx(y):
+    return lib().hko_erfcx(float(y))
+
+
+def faddeeva(x, y):
+    """Complex w(x + iy) for y >= 0 (arrays)."""
+    x = _f64(np.atleast_1d(x)); y = _f64(np.broadcast_to(y, x.shape))
+    re = np.empty_like(x); im = np.empty_like(x)
+    lib().hko_faddeeva_batch(x.size, _p(x), _p(y), _p(re), _p(im))
+    return re + 1j * im
+
+
+def w916(x, y, tol=1e-17):
+    re = ctypes.c_double(); im = ctypes.c_double()
+    lib().hko_w916(float(x), float(y), tol, ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def wcf(x, y, depth):
+    re = ctypes.c_double(); im = ctypes.c_double()
+    lib().hko_wcf(float(x), float(y), int(depth), ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def _iso_arrays(lines):
+    iso = lines.get("iso")
+    iso = None if iso is None else np.ascontiguousarray(iso, dtype=np.int32)
+    tabs = [_f64(lines[k]) for k in ("iso_mass", "iso_ab", "iso_sscale", "iso_Q")]
+    return iso, tabs
+
+
+def line_params(lines, T, P, cut_mode, cut):
+    """Line state at (T, P[atm]): dict nu, S, ialphaD, y, hw."""
+    nl = len(lines["nu"])
+    cols = [_f64(lines[k]) for k in ("nu", "S", "EL", "A", "delta", "vy", "n")]
+    iso, tabs = _iso_arrays(lines)
+    out = {k: np.empty(nl) for k in ("nu", "S", "ialphaD", "y", "hw")}
+    lib().hko_line_params(nl, *[_p(c) for c in cols],
+                          None if iso is None else _p(iso, _I32),
+                          *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
+                          float(cut), *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
+    return out
+
+
+def window(nu, hw, numin, dnu, Nx):
+    lo = ctypes.c_int64(); hi = ctypes.c_int64()
+    lib().hko_window(float(nu), float(hw), float(numin), float(dnu), int(Nx),
+                     ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
+                 nthreads=0):
+    """K at grid points jidx (all points if None); lines sorted by nu."""
+    cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
+    nl = cols[0].size
+    if search_radius is None:
+        hw = cols[4]
+        search_radius = float(hw[np.isfinite(hw)].max()) if nl else 0.0
+    if jidx is None:
+        npts = int(Nx); jp = None
+    else:
+        jidx = np.ascontiguousarray(jidx, dtype=np.int64); npts = jidx.size
+        jp = _p(jidx, _I64)
+    K = np.empty(npts)
+    lib().hko_voigt_points(nl, *[_p(c) for c in cols], float(search_radius),
+                           float(numin), float(dnu), int(Nx), float(kmin), npts, jp,
+                           _p(K), int(nthreads))
+    return K
+
+
+def voigt_grad(lines, T, P, cut_mode, cut, numin, dnu, Nx, gK, nthreads=0):
+    """(dL/dT, dL/dP, dL/dS_i, dL/dvy_i) for L = sum_j gK_j K_j."""
+    nl = len(lines["nu"])
+    cols = [_f64(lines[k]) for k in ("nu", "S", "EL", "A", "delta", "vy", "n")]
+    iso, tabs = _iso_arrays(lines)
+    gK = _f64(gK)
+    gS = np.empty(nl); gv = np.empty(nl); gTP = np.empty(2)
+    lib().hko_voigt_grad(nl, *[_p(c) for c in cols],
+                         None if iso is None else _p(iso, _I32),
+                         *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
+                         float(cut), float(numin), float(dnu), int(Nx), _p(gK),
+                         _p(gS), _p(gv), _p(gTP), int(nthreads))
+    return gTP[0], gTP[1], gS, gv
+
+
+def bin_segments(numin, numax, dnu, Nx, nbins):
+    seg = np.empty(nbins + 2, dtype=np.int64)
+    lib().hko_bin_segments(float(numin), float(numax), float(dnu), int(Nx), int(nbins),
+                           _p(seg, _I64))
+    return seg
+
+
+def sort_segments(K, seg):
+    K = _f64(K).copy()
+    seg = np.ascontiguousarray(seg, dtype=np.int64)
+    lib().hko_sort_segments(_p(K), int(seg.size - 1), _p(seg, _I64))
+    return K
+
+
+def qr(Nxb, nC):
+    V = np.empty(Nxb * nC); C = np.empty(nC); D = np.empty(nC)
+    lib().hko_qr(int(Nxb), int(nC), _p(V), _p(C), _p(D))
+    return V.reshape(nC, Nxb), C, D
+
+
+def ktable(Ksorted, Nxb, nbins, nC, kmin, unit_scale=1.0):
+    """Coefficients [nbins, nC] and Nxmin [nbins] from a per-bin sorted K."""
+    K = _f64(Ksorted).copy()
+    coeff = np.empty(nbins * nC); nxmin = np.empty(nbins, dtype=np.intc)
+    lib().hko_ktable(_p(K), K.size, int(Nxb), int(nbins), int(nC), float(kmin),
+                     float(unit_scale), _p(coeff), _p(nxmin, _INT))
+    return coeff.reshape(nbins, nC), nxmin.astype(np.int64)
+
+
+def reconstruct(coeff, nxmin, Nxb, kmin, unit_scale=1.0):
+    coeff = _f64(coeff); nbins, nC = coeff.shape
+    nx = np.ascontiguousarray(nxmin, dtype=np.intc)
+    Ky = np.empty(nbins * Nxb)
+    lib().hko_reconstruct(_p(coeff), _p(nx, _INT), int(Nxb), int(nbins), int(nC),
+                          float(kmin), float(unit_scale), _p(Ky))
+    return Ky

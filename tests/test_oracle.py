@@ -1,0 +1,304 @@
+"""Pins the FP64 host oracle (oracle/hk_oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign,
+wrong index, transposed operand) fails it: mpmath golden vectors, closed
+forms, brute force on tiny inputs, and the reference's own example outputs
+(plots/p004, p005, p009, copied into tests/golden).
+"""
+import hashlib
+from fractions import Fraction
+import math
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+C2 = 6.62606957e-27 * 2.99792458e10 / 1.3806489e-16   # h c / kB, define.h:39-41
+T0 = 296.0
+
+
+# ---------------------------------------------------------------- Faddeeva
+def test_faddeeva_vs_mpmath_golden(golden_dir):
+    d = np.loadtxt(os.path.join(golden_dir, "faddeeva_mpmath.csv"), delimiter=",")
+    x, y, re, im = d.T
+    w = O.faddeeva(x, y)
+    nz = re != 0.0            # exp(-x^2) below the double range reads as 0
+    assert np.all(w.real[~nz] < 1e-300)
+    relre = np.abs(w.real[nz] - re[nz]) / np.abs(re[nz])
+    assert relre.max() < 1e-13, relre.max()
+    # Im w: absolute accuracy relative to |w| (Im w -> 0 at x -> 0)
+    absim = np.abs(w.imag - im) / np.abs(re + 1j * im)
+    assert absim.max() < 1e-13
+
+
+def test_faddeeva_closed_forms():
+    ys = np.array([0.0, 1e-6, 0.1, 1.0, 4.9, 5.1, 30.0, 1e4])
+    for y in ys:
+        ref = float(mp.exp(mp.mpf(y) ** 2) * mp.erfc(mp.mpf(y)))
+        assert abs(O.erfcx(y) - ref) <= 2e-15 * ref
+        assert abs(O.faddeeva(0.0, y)[0].real - ref) <= 2e-15 * ref      # w(iy) = erfcx(y)
+    xs = np.array([0.0, 0.3, 1.0, 3.0, 6.0, 9.0, 9.99, 10.01, 20.0])
+    w = O.faddeeva(xs, 0.0)
+    np.testing.assert_allclose(w.real, np.exp(-xs * xs), rtol=1e-14, atol=0)  # Re w(x,0)
+    # Dawson: Im w(x,0) = 2/sqrt(pi) D(x)
+    daw = np.array([float(2 / mp.sqrt(mp.pi) * mp.sqrt(mp.pi) / 2 * mp.exp(-mp.mpf(v) ** 2)
+                          * mp.erfi(mp.mpf(v))) for v in xs])
+    np.testing.assert_allclose(w.imag, daw, rtol=1e-13, atol=1e-300)
+    # symmetry w(-x+iy) = conj w(x+iy)
+    a = O.faddeeva(np.array([2.5, 50.0]), 0.7); b = O.faddeeva(np.array([-2.5, -50.0]), 0.7)
+    np.testing.assert_allclose(b, np.conj(a), rtol=1e-15)
+
+
+def test_faddeeva_normalisation():
+    # integral of Re w(x, y) dx over the real line = sqrt(pi)
+    for y in (0.05, 1.0, 20.0):
+        L = 2.0e4
+        x = np.linspace(-L, L, 4_000_001)
+        f = O.faddeeva(x, y).real
+        integral = np.trapezoid(f, x) + 2 * y / (math.sqrt(math.pi) * L)  # tails beyond +-L
+        assert abs(integral - math.sqrt(math.pi)) < 1e-9 * math.sqrt(math.pi)
+
+
+def test_faddeeva_methods_agree_on_overlap():
+    # Algorithm 916 and the continued fraction are independent methods
+    for x, y in ((8.0, 6.0), (10.0, 1e-3), (3.0, 12.0), (15.0, 0.2)):
+        a = O.w916(x, y); b = O.wcf(x, y, 40)
+        assert abs(a.real - b.real) <= 1e-13 * abs(b.real)
+
+
+# ---------------------------------------------------------------- line parameters
+def _lines(n, seed=1, delta=0.0, A=0.0):
+    rng = np.random.default_rng(seed)
+    nu = np.sort(rng.uniform(100.0, 3000.0, n))
+    return dict(nu=nu, S=10 ** rng.uniform(-25, -19, n), EL=rng.uniform(0, 20000, n),
+                A=np.full(n, A), delta=np.full(n, delta), vy=rng.uniform(0.01, 0.1, n),
+                n=rng.uniform(0.3, 0.8, n), iso=None,
+                iso_mass=np.array([18.010565 / 6.0221412927e23]), iso_ab=np.array([0.9973]),
+                iso_sscale=np.array([1.0]), iso_Q=np.array([174.58]))
+
+
+def test_line_params_closed_forms():
+    L = _lines(200)
+    m = L["iso_mass"][0]
+    ia0 = 2.99792458e10 * math.sqrt(m / (2 * 1.3806489e-16 * T0))
+    st = O.line_params(L, T0, 1.0, 0, 25.0)
+    # at T0 the Boltzmann and stimulated-emission ratios are exactly 1
+    S_expect = L["S"] / 174.58 * 0.9973 * (ia0 / L["nu"])
+    np.testing.assert_allclose(st["S"], S_expect, rtol=1e-14)
+    np.testing.assert_allclose(st["ialphaD"] * L["nu"], ia0, rtol=1e-15)
+    np.testing.assert_allclose(st["y"], L["vy"] * st["ialphaD"], rtol=1e-15)
+    assert np.all(st["hw"] == 25.0)
+    # temperature scaling: alpha_D ~ sqrt(T); alpha_L ~ (T0/T)^n
+    T = 1500.0
+    s2 = O.line_params(L, T, 3.0, 0, 25.0)
+    np.testing.assert_allclose(st["ialphaD"] / s2["ialphaD"], math.sqrt(T / T0), rtol=1e-14)
+    aL = s2["y"] / s2["ialphaD"]
+    np.testing.assert_allclose(aL, L["vy"] * 3.0 * (T0 / T) ** L["n"], rtol=1e-14)
+    # S(T)/S(T0): Boltzmann and stimulated-emission factors
+    ratio = (np.exp(-C2 * L["EL"] / T + C2 * L["EL"] / T0)
+             * (1 - np.exp(-C2 * L["nu"] / T)) / (1 - np.exp(-C2 * L["nu"] / T0)))
+    np.testing.assert_allclose(s2["S"] / st["S"], ratio * s2["ialphaD"] / st["ialphaD"], rtol=1e-12)
+    # cut modes: 1 = Lorentz widths, 2 = Doppler widths
+    s1 = O.line_params(L, T, 3.0, 1, 100.0)
+    np.testing.assert_allclose(s1["hw"], 100.0 * aL, rtol=1e-15)
+    s3 = O.line_params(L, T, 3.0, 2, 50.0)
+    np.testing.assert_allclose(s3["hw"], 50.0 / s3["ialphaD"], rtol=1e-15)
+
+
+def test_line_params_shift_and_natural_width():
+    L = _lines(50, delta=-0.01, A=2.0e7)
+    st = O.line_params(L, 800.0, 2.0, 0, 25.0)
+    np.testing.assert_allclose(st["nu"], L["nu"] - 0.02, rtol=1e-15)
+    aL = st["y"] / st["ialphaD"]
+    expect = L["vy"] * 2.0 * (T0 / 800.0) ** L["n"] + 2.0e7 / (4 * math.pi * 2.99792458e10)
+    np.testing.assert_allclose(aL, expect, rtol=1e-14)
+    L["nu"][0] = 0.02       # shifted to exactly 0 -> zero line
+    st = O.line_params(L, 800.0, 2.0, 0, 25.0)
+    assert st["S"][0] == 0 and st["ialphaD"][0] == 0 and st["hw"][0] < 0
+
+
+# ---------------------------------------------------------------- windows
+def test_window_brute_force():
+    rng = np.random.default_rng(3)
+    numin, dnu, Nx = 10.0, 0.01, 5000
+    j = np.arange(Nx, dtype=np.float64)
+    x = numin + j * dnu   # fma equals mul+add here to within 1 ulp; check both sides
+    for _ in range(300):
+        nu = rng.uniform(0.0, 70.0)
+        hw = rng.choice([rng.uniform(0.0, 5.0), 1e30, 0.0, dnu * rng.integers(1, 50)])
+        lo, hi = O.window(nu, hw, numin, dnu, Nx)
+        inside = np.abs(x - nu) < hw
+        idx = np.nonzero(inside)[0]
+        if idx.size == 0:
+            assert hi == lo
+        else:
+            # a point exactly on the cut edge may flip with fma rounding: allow 1
+            assert abs(lo - idx[0]) <= 1 and abs(hi - (idx[-1] + 1)) <= 1
+            assert hi - lo >= idx.size - 2
+
+
+# ---------------------------------------------------------------- Voigt sum
+def _state(n=3, seed=0):
+    rng = np.random.default_rng(seed)
+    nu = np.sort(rng.uniform(1000.2, 1000.8, n))
+    return dict(nu=nu, S=rng.uniform(0.5, 2.0, n), ialphaD=rng.uniform(5.0, 40.0, n),
+                y=rng.uniform(1e-3, 3.0, n), hw=np.full(n, 0.3))
+
+
+def test_voigt_points_vs_mpmath_brute_force():
+    st = _state(3)
+    numin, dnu, Nx = 1000.0, 0.02, 50
+    K = O.voigt_points(st, numin, dnu, Nx, kmin=1e-9)
+    mp.mp.dps = 30
+    for j in range(0, Nx, 7):
+        xj = mp.mpf(float(Fraction(j) * Fraction(dnu) + Fraction(numin)))  # fma(j, dnu, numin)
+        tot = mp.mpf("1e-9")
+        for i in range(3):
+            d = xj - mp.mpf(st["nu"][i])
+            if abs(d) < st["hw"][i]:
+                z = mp.mpc(d * st["ialphaD"][i], st["y"][i])
+                tot += st["S"][i] * (mp.exp(-z * z) * mp.erfc(-1j * z)).real / mp.sqrt(mp.pi)
+        assert abs(K[j] - float(tot)) <= 1e-13 * float(tot)
+
+
+def test_voigt_points_invariants():
+    st = _state(5, seed=2)
+    numin, dnu, Nx = 1000.0, 0.005, 400
+    K1 = O.voigt_points(st, numin, dnu, Nx)
+    st2 = dict(st); st2["S"] = st["S"] * 3.0
+    np.testing.assert_allclose(O.voigt_points(st2, numin, dnu, Nx), 3.0 * K1, rtol=1e-15)  # linear
+    # translation by whole grid steps
+    m = 37
+    st3 = dict(st); st3["nu"] = st["nu"] + m * dnu
+    K3 = O.voigt_points(st3, numin, dnu, Nx + m)
+    np.testing.assert_allclose(K3[m + 60:m + 340], K1[60:340], rtol=1e-9)
+    # symmetry of one line centred on a grid point
+    one = dict(nu=np.array([1000.0 + 200 * dnu]), S=np.array([1.0]), ialphaD=np.array([30.0]),
+               y=np.array([0.4]), hw=np.array([1.0]))
+    K = O.voigt_points(one, numin, dnu, Nx)
+    np.testing.assert_allclose(K[200 - 150:200], K[200 + 150:200:-1], rtol=1e-12)
+    # subset of points equals the full evaluation
+    jj = np.array([0, 5, 199, 200, 399])
+    np.testing.assert_array_equal(O.voigt_points(st, numin, dnu, Nx, jidx=jj), K1[jj])
+
+
+def test_voigt_points_normalisation():
+    # one uncut line on a wide grid: sum_j K_j dnu = S minus the Lorentz tails
+    aD = 0.05
+    for y in (0.02, 1.0):
+        st = dict(nu=np.array([500.0]), S=np.array([1.0 / aD]), ialphaD=np.array([1 / aD]),
+                  y=np.array([y]), hw=np.array([1e30]))
+        numin, dnu, Nx = 300.0, 0.002, 200_000
+        K = O.voigt_points(st, numin, dnu, Nx)
+        aL = y * aD
+        tails = aL / math.pi * (1 / 200.0 + 1 / 200.0)
+        # state S carries 1/alpha_D, so S = 1/aD is a line of unit strength
+        assert abs(K.sum() * dnu - (1 - tails)) < 1e-8
+
+
+# ---------------------------------------------------------------- gradient
+def test_voigt_grad_vs_central_differences():
+    L = _lines(6, seed=4)
+    L["nu"] = np.sort(np.random.default_rng(5).uniform(1000.0, 1001.0, 6))
+    L["S"] = np.full(6, 1e-20)
+    L["EL"] = np.linspace(0.0, 300.0, 6)        # comparable line contributions
+    numin, dnu, Nx = 999.5, 0.002, 1000
+    T, P = 900.0, 0.7
+    gK = np.random.default_rng(6).uniform(0.0, 1.0, Nx)
+
+    def loss(LL, TT, PP):
+        st = O.line_params(LL, TT, PP, 0, 0.8)
+        return float(np.dot(gK, O.voigt_points(st, numin, dnu, Nx)))
+
+    gT, gP, gS, gv = O.voigt_grad(L, T, P, 0, 0.8, numin, dnu, Nx, gK)
+    h = 1e-5
+    fdT = (loss(L, T * (1 + h), P) - loss(L, T * (1 - h), P)) / (2 * h * T)
+    fdP = (loss(L, T, P * (1 + h)) - loss(L, T, P * (1 - h))) / (2 * h * P)
+    assert abs(gT - fdT) <= 1e-6 * abs(fdT)
+    assert abs(gP - fdP) <= 1e-6 * abs(fdP)
+    for i in (0, 3, 5):
+        for key, g in (("S", gS), ("vy", gv)):
+            Lp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in L.items()}
+            Lm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in L.items()}
+            Lp[key][i] *= 1 + h; Lm[key][i] *= 1 - h
+            fd = (loss(Lp, T, P) - loss(Lm, T, P)) / (2 * h * L[key][i])
+            assert abs(g[i] - fd) <= 1e-6 * abs(fd), (key, i, g[i], fd)
+
+
+# ---------------------------------------------------------------- k-table
+def _cbin(path):
+    return np.loadtxt(path)
+
+
+def test_sort_matches_p004_and_p009(golden_dir):
+    K = np.load(os.path.join(golden_dir, "p004_K.npy"))
+    seg = O.bin_segments(0.0, 30000.0, 1.0, 30000, 1)
+    assert list(seg) == [0, 30000, 30000]
+    s = O.sort_segments(K, seg[:2])
+    ref = open(os.path.join(golden_dir, "p004_sorted.sha256")).read().split()[0]
+    assert hashlib.sha256(s.tobytes()).hexdigest() == ref
+    for run in ("p1", "p3"):
+        K = np.load(os.path.join(golden_dir, "p009_%s_K.npz" % run))["K"]
+        seg = O.bin_segments(24400.0, 24500.0, 0.001, 100000, 5)
+        assert list(seg) == [0, 20000, 40000, 60000, 80000, 100000, 100000]
+        s = O.sort_segments(K, seg[:6])
+        ref = open(os.path.join(golden_dir, "p009_%s_sorted.sha256" % run)).read().split()[0]
+        assert hashlib.sha256(s.tobytes()).hexdigest() == ref
+
+
+def test_ktable_matches_p004(golden_dir):
+    K = np.load(os.path.join(golden_dir, "p004_K.npy"))
+    Ks = np.sort(K)
+    coeff, nxmin = O.ktable(Ks, 30000, 1, 20, 0.0)
+    ref = _cbin(os.path.join(golden_dir, "p004_Out_r_cbin.dat"))
+    assert ref[0] == 0.0 and nxmin[0] == 4533 and ref[1] == min(nxmin[0] / 29999.0, 1.0)
+    np.testing.assert_allclose(coeff[0], ref[2:], rtol=2e-12, atol=1e-13)
+    Ky = O.reconstruct(coeff, nxmin, 30000, 0.0)
+    Kref = np.load(os.path.join(golden_dir, "p004_Out_r_bin_K.npy"))
+    np.testing.assert_allclose(Ky, Kref, rtol=1e-12)
+
+
+def test_ktable_matches_p005(golden_dir):
+    Ks = np.loadtxt(os.path.join(golden_dir, "p005_Out_i_bin.dat"), usecols=(1,))
+    coeff, nxmin = O.ktable(Ks, 500, 1, 20, 1e-7)
+    ref = _cbin(os.path.join(golden_dir, "p005_Out_r_cbin.dat"))
+    assert nxmin[0] == 426 and ref[1] == 426 / 499.0
+    np.testing.assert_allclose(coeff[0], ref[2:], rtol=1e-11, atol=1e-13)
+    Ky = O.reconstruct(coeff, nxmin, 500, 1e-7)
+    Kref = np.loadtxt(os.path.join(golden_dir, "p005_Out_r_bin.dat"), usecols=(1,))
+    np.testing.assert_allclose(Ky, Kref, rtol=1e-12)
+
+
+def test_ktable_quirk_is_pinned(golden_dir):
+    # Planting the textbook NC reflections (instead of NC-1) must break p005.
+    Ks = np.loadtxt(os.path.join(golden_dir, "p005_Out_i_bin.dat"), usecols=(1,))
+    ref = _cbin(os.path.join(golden_dir, "p005_Out_r_cbin.dat"))
+    n = 426; Nxb = 500
+    s = 1.0 - n / (Nxb - 1.0)
+    ii = np.minimum(n + s * np.arange(Nxb), 0.999999 * (Nxb - 1))
+    ii = np.where(n + s * np.arange(Nxb) >= Nxb - 1, 0.999999 * (Nxb - 1), n + s * np.arange(Nxb))
+    il = (ii / Nxb * Nxb).astype(int)
+    b = np.log(Ks[il] + (Ks[il + 1] - Ks[il]) * (ii - il))
+    x = np.arange(Nxb) * 2.0 / (Nxb - 1.0) - 1.0
+    V = np.polynomial.chebyshev.chebvander(x, 19)
+    c_textbook = np.linalg.lstsq(V, b, rcond=None)[0]
+    assert np.max(np.abs(c_textbook - ref[2:]) / np.abs(ref[2:])) > 1e-3
+
+
+def test_findcut_rules():
+    # non-last bin: Nxmin = count(K <= kmin); last bin: K[Nx-3] <= kmin -> Nxb
+    Nxb, nbins = 10, 3
+    K = np.concatenate([np.r_[np.zeros(4), np.linspace(1, 2, 6)],
+                        np.zeros(10),
+                        np.r_[np.zeros(8), 1.0, 2.0]])
+    _, nxmin = O.ktable(K.copy(), Nxb, nbins, 4, 0.0)
+    assert list(nxmin) == [4, 10, 10]
+    K[20:] = np.r_[np.zeros(5), np.linspace(1, 2, 5)]
+    _, nxmin = O.ktable(K.copy(), Nxb, nbins, 4, 0.0)
+    assert list(nxmin) == [4, 10, 5]
+    K[:10] = np.linspace(1, 2, 10)
+    _, nxmin = O.ktable(K.copy(), Nxb, nbins, 4, 0.0)
+    assert nxmin[0] == 0
