@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the line-by-line path on one MI355X.
+
+Workload (default): 1e7 synthetic lines, grid 0-30000 cm^-1 at 0.001 (3e7
+points), T = 2000 K, P = 10 bar, cut 25 cm^-1, 300 bins x 20 Chebyshev
+coefficients -- the precision-at-scale configuration.  One step = line
+parameters + FP32-compensated Voigt sum + k-table over the whole grid, with
+the line list already resident in HBM.  value = algorithmic Voigt
+evaluations (line-point pairs inside the cut windows, counted exactly by the
+library) per second of step time.
+
+Also reported: the plain FP64 form's evaluations/s on the same input, the
+k-table points/s, FP32C-vs-FP64 max relative error on the full grid, the
+Voigt kernel's roofline fraction (fixed per-region operation model, see
+FLOPS_PER_EVAL) and a CPU baseline (the FP64 oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "helios-k_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+# Operations per evaluation of the FP64 plain method, counted on its formulas
+# with divisions, exponentials and trigonometric calls weighted 4, for the
+# reference's regions (A: r2 >= 1e6 one-term Lorentz wing; B: 100 <= r2 < 1e6
+# rational; C: r2 < 100 Algorithm 916 with ~9 series terms).
+FLOPS_PER_EVAL = {"A": 10.0, "B": 22.0, "C": 400.0}
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
+PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-lines", type=int, default=10000,
+                    help="lines of the CPU-baseline sample (first lines by generator index)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(the path shards by wavenumber range or (T,P) with no exchange; see DESIGN.md)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from heliosk import configs
+    from heliosk import engine as E
+
+    w = configs.WORKLOADS[args.workload]
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log("bench: workload %s (%s)" % (args.workload, w.note))
+    t0 = time.time()
+    arr = configs.species_arrays(w, 0)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    T, Pbar = w.tp[0]
+    P = Pbar * configs.POBAR
+    grid = E.make_grid(w.numin, w.numax, w.dnu)
+    kt = E.KTable(grid, w.numax, w.nbins, w.nC)
+    st = E.State(sp.nlines)
+    K = torch.empty(grid.nx, dtype=torch.float64, device=dev)
+    nbytes = E.lib().hk_voigt_workspace_bytes(E.ctypes.byref(grid), E.ctypes.byref(E._tune(0)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    log("bench: setup %.1f s (%d lines, %d points)" % (time.time() - t0, sp.nlines, grid.nx))
+
+    E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut, state=st)
+    counts = E.pair_count(st, grid)
+    evals = counts[0]
+    region = {"A": counts[1], "B": counts[2], "C": counts[3]}
+    flops = sum(region[r] * FLOPS_PER_EVAL[r] for r in region)
+    log("bench: %.4e evaluations per step, regions A/B/C = %s" % (evals, region))
+
+    def step(mode, ev=None):
+        E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut, state=st)
+        if ev is not None:
+            ev[0].record()
+        E.voigt(st, grid, 0.0, mode, K=K, workspace=ws)
+        if ev is not None:
+            ev[1].record()
+        return kt(K, kmin=0.0)
+
+    mode = E.HK_VOIGT_FP32C
+    for i in range(args.warmup):
+        step(mode)
+        torch.cuda.synchronize()
+        log("bench: warmup step %d done" % i)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        coeff, nxmin = step(mode, evs[i])
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t_start) / args.steps
+    voigt_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    log("bench: FP32C step %.3f ms, Voigt kernel %.3f ms" % (t_step * 1e3, voigt_ms))
+    K32 = K.clone()
+
+    # k-table alone (sort + fit) on the resident K
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        kt(K32, kmin=0.0)
+    e1.record()
+    torch.cuda.synchronize()
+    ktable_ms = e0.elapsed_time(e1) / 3
+
+    # plain FP64 form on the same input
+    fp64_ms = None
+    if args.fp64_steps > 0:
+        E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, K=K, workspace=ws)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.fp64_steps):
+            E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, K=K, workspace=ws)
+        e1.record()
+        torch.cuda.synchronize()
+        fp64_ms = e0.elapsed_time(e1) / args.fp64_steps
+        log("bench: FP64 Voigt kernel %.3f ms" % fp64_ms)
+        Kr = K
+        tau = 1e-12 * float(Kr.max())
+        rel = float(torch.max(torch.abs(K32 - Kr) / torch.clamp(torch.abs(Kr), min=tau)))
+    else:
+        rel = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, arr, T, P, grid, args.cpu_lines)
+
+    achieved = flops / (voigt_ms * 1e-3) / 1e12
+    out = {
+        "metric": "Voigt evals/s + k-table pts/s, batched FP32-comp vs plain FP64 at <=1e-5 rel err",
+        "value": evals / t_step,
+        "unit": "Voigt evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (compensated) profile, f64 line params and k-fit",
+        "data": "synthetic (seeded generator, %d lines)" % sp.nlines,
+        "config": {"workload": "%s: %s" % (w.name, w.note), "lines": sp.nlines,
+                   "grid_points": grid.nx, "T_K": T, "P_bar": Pbar, "cut_cm-1": w.cut,
+                   "bins": w.nbins, "nC": w.nC, "parallelism": "single GPU"},
+        "voigt_evals_per_step": evals,
+        "voigt_regions_ABC": [region["A"], region["B"], region["C"]],
+        "voigt_kernel_ms": voigt_ms,
+        "voigt_evals_per_s_kernel": evals / (voigt_ms * 1e-3),
+        "fp64_voigt_kernel_ms": fp64_ms,
+        "fp64_voigt_evals_per_s": evals / (fp64_ms * 1e-3) if fp64_ms else None,
+        "speedup_fp32c_vs_fp64": fp64_ms / voigt_ms if fp64_ms else None,
+        "fp32c_vs_fp64_max_rel_err": rel,
+        "ktable_ms": ktable_ms,
+        "ktable_points_per_s": grid.nx / (ktable_ms * 1e-3),
+        "ktable_hbm_GBs_algorithmic": 8.0 * grid.nx / (ktable_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "model": "ops/eval A=%g B=%g C=%g (FP64 formulas, div/exp/trig x4)"
+                              % (FLOPS_PER_EVAL["A"], FLOPS_PER_EVAL["B"], FLOPS_PER_EVAL["C"])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(w, arr, T, P, grid, nlines):
+    """The FP64 oracle (line parameters + Voigt sum over the full grid) on the
+    first `nlines` lines by generator index, timed on this host's cores."""
+    import numpy as np
+    from heliosk import configs
+    from oracle import oracle as O
+    ncores = min(16, os.cpu_count() or 1)
+    sub = {k: (v[:nlines] if isinstance(v, np.ndarray) else v) for k, v in arr.items()}
+    ol = configs.oracle_lines(sub)
+    t0 = time.perf_counter()
+    ost = O.line_params(ol, T, P, 0, w.cut)
+    K = O.voigt_points(ost, grid.numin, grid.dnu, grid.nx, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    nu = ost["nu"]
+    # evaluations of the sample: window points of each line
+    j = (nu[:, None] + np.array([-w.cut, w.cut])[None, :] - grid.numin) / grid.dnu
+    lo = np.clip(np.floor(j[:, 0]) + 1, 0, grid.nx); hi = np.clip(np.floor(j[:, 1]) + 1, 0, grid.nx)
+    ev = float(np.sum(np.maximum(hi - lo, 0)))
+    log("bench: CPU oracle %d lines, %.3e evals in %.1f s on %d threads" % (nlines, ev, dt, ncores))
+    return {"value": ev / dt, "unit": "Voigt evaluations/s", "cores": ncores, "kind": "port",
+            "sample": "FP64 oracle (Algorithm 916 + continued fraction, compensated sum) over the "
+                      "full %d-point grid for the first %d lines of the workload" % (grid.nx, nlines),
+            "seconds": dt, "checksum": float(np.sum(K))}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+// hk_adjoint.hip -- adjoint of the Voigt sum.
+//
+// For L = sum_j gK_j K_j with K_j = kmin + sum_i S_i Re w(x_ij, y_i)/sqrt(pi)
+// (the forward sum of hk_voigt_accumulate), one wavefront per line recomputes
+// the line parameters (FP64, as hk_line_params) and the complex profile w on
+// the line's cut window -- nothing of the forward pass is stored -- and
+// accumulates
+//   dL/dS_i   = F_i  sum_j gK_j Re w / sqrt(pi)              (S = S_file)
+//   dL/dvy_i  = S_i  sum_j gK_j dRe w/dy / sqrt(pi) * dy/dvy
+//   dL/dT, dL/dP = sum_i S_i sum_j gK_j [Re w dlnS + dRe w/dx dx + dRe w/dy dy]
+// with w'(z) = -2 z w + 2i/sqrt(pi):  dRe w/dx = -2 (x Re w - y Im w),
+// dRe w/dy = 2 (x Im w + y Re w) - 2/sqrt(pi).  The partition function is
+// held fixed.  Per-line T and P terms are summed afterwards by one
+// workgroup in a fixed order, so the result is deterministic.
+#include <math.h>
+
+#include "hk_device.h"
+#include "hk_internal.h"
+
+namespace {
+
+struct IsoTabA {
+    double mass[HK_MAX_ISO], ab[HK_MAX_ISO], ss[HK_MAX_ISO], Q[HK_MAX_ISO];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void adjoint_kernel(
+    const double *__restrict__ nu0, const double *__restrict__ S0,
+    const double *__restrict__ EL, const double *__restrict__ A,
+    const double *__restrict__ delta, const double *__restrict__ vy0,
+    const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTabA tab,
+    int64_t nl, double T, double P, int cut_mode, double cut, double numin, double dnu,
+    int64_t nx, const double *__restrict__ gK, double *__restrict__ gS,
+    double *__restrict__ gvy, double *__restrict__ gTl, double *__restrict__ gPl) {
+    using namespace hk;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nl) return;
+    const double c = kH * kC / kKB;
+    const int k = iso ? min(max(iso[i], 0), HK_MAX_ISO - 1) : 0;
+    const double ialphaD_h = kC * sqrt(tab.mass[k] / (2.0 * kKB * T));
+    const double F0 = 1.0 / tab.Q[k] * tab.ab[k] * tab.ss[k];
+    const double nu = fma(delta[i], P, nu0[i]);
+    if (nu == 0.0) {
+        if (lane == 0) { gS[i] = 0.0; gvy[i] = 0.0; gTl[i] = 0.0; gPl[i] = 0.0; }
+        return;
+    }
+    const double ia = ialphaD_h / nu;
+    const double E = EL[i];
+    const double B = exp(-E * c / T + E * c / kT0);
+    const double eT = exp(-c * nu / T), e0 = exp(-c * nu / kT0);
+    const double Rr = (1.0 - eT) / (1.0 - e0);
+    const double F = F0 * B * Rr * ia;
+    const double S = S0[i] * F;
+    const double pw = pow(kT0 / T, nexp[i]);
+    const double aLp = vy0[i] * P * pw;
+    const double aL = aLp + A[i] / (4.0 * kPi * kC);
+    const double y = aL * ia;
+    const double hw = cut_mode == 1 ? cut * aL : (cut_mode == 2 ? cut / ia : cut);
+    const double dlnR_dT = -eT * (c * nu / (T * T)) / (1.0 - eT);
+    const double dlnR_dnu = eT * (c / T) / (1.0 - eT) - e0 * (c / kT0) / (1.0 - e0);
+    const double dlnS_dT = c * E / (T * T) + dlnR_dT - 0.5 / T;
+    const double dlnS_dP = delta[i] * (dlnR_dnu - 1.0 / nu);
+    const double dia_dT = -0.5 * ia / T, dia_dP = -ia * delta[i] / nu;
+    const double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * pw;
+    const double dy_dT = daL_dT * ia + aL * dia_dT;
+    const double dy_dP = daL_dP * ia + aL * dia_dP;
+    const double dy_dvy = P * pw * ia;
+    int64_t jlo, jhi;
+    cut_window(nu, hw, numin, dnu, nx, jlo, jhi);
+    double sre = 0.0, sdy = 0.0, sT = 0.0, sP = 0.0;
+    for (int64_t j = jlo + lane; j < jhi; j += 64) {
+        const double g = gK[j];
+        const double d = fma((double)j, dnu, numin) - nu;
+        const double x = d * ia;
+        double re, im;
+        w64(x, y, re, im);
+        const double dRe_dx = -2.0 * (x * re - y * im);
+        const double dRe_dy = 2.0 * (x * im + y * re) - 2.0 * kISqrtPi;
+        const double dx_dT = d * dia_dT;
+        const double dx_dP = -delta[i] * ia + d * dia_dP;
+        sre = fma(g, re, sre);
+        sdy = fma(g, dRe_dy, sdy);
+        sT = fma(g, re * dlnS_dT + dRe_dx * dx_dT + dRe_dy * dy_dT, sT);
+        sP = fma(g, re * dlnS_dP + dRe_dx * dx_dP + dRe_dy * dy_dP, sP);
+    }
+    sre = wave_sum(sre); sdy = wave_sum(sdy); sT = wave_sum(sT); sP = wave_sum(sP);
+    if (lane == 0) {
+        gS[i] = F * sre * kISqrtPi;
+        gvy[i] = S * sdy * dy_dvy * kISqrtPi;
+        gTl[i] = S * sT * kISqrtPi;
+        gPl[i] = S * sP * kISqrtPi;
+    }
+}
+
+__global__ __launch_bounds__(1024) void sum2_kernel(const double *__restrict__ a,
+                                                    const double *__restrict__ b, int64_t n,
+                                                    double *__restrict__ out) {
+    __shared__ double sa[1024], sb[1024];
+    double x = 0.0, y = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) { x += a[i]; y += b[i]; }
+    sa[threadIdx.x] = x; sb[threadIdx.x] = y;
+    __syncthreads();
+    for (int k = 512; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { sa[threadIdx.x] += sa[threadIdx.x + k]; sb[threadIdx.x] += sb[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sa[0]; out[1] = sb[0]; }
+}
+
+}  // namespace
+
+extern "C" size_t hk_voigt_adjoint_workspace_bytes(int64_t nlines) {
+    if (nlines < 0) return 0;
+    return (size_t)(2 * nlines + 2) * sizeof(double) + 256;
+}
+
+extern "C" int hk_voigt_adjoint(const hk_lines *L, const hk_tp *tp, const hk_cut *cut,
+                                const hk_grid *g, const double *gK, double *gS, double *gvy,
+                                double *gTP, void *ws, size_t ws_bytes, hk_stream_t stream) {
+    HK_CHECK_ARG(L && tp && cut && g && gK && gS && gvy && gTP && ws,
+                 "hk_voigt_adjoint: null argument");
+    HK_CHECK_ARG(L->niso >= 1 && L->niso <= HK_MAX_ISO, "hk_voigt_adjoint: niso %d", L->niso);
+    HK_CHECK_ARG(g->nx > 0 && g->dnu > 0.0, "hk_voigt_adjoint: bad grid");
+    if (ws_bytes < hk_voigt_adjoint_workspace_bytes(L->nlines)) {
+        hk::set_error("hk_voigt_adjoint: workspace %zu bytes, %zu needed", ws_bytes,
+                      hk_voigt_adjoint_workspace_bytes(L->nlines));
+        return HK_ERR_WORKSPACE;
+    }
+    IsoTabA tab;
+    for (int k = 0; k < HK_MAX_ISO; ++k) {
+        bool used = k < L->niso;
+        tab.mass[k] = used ? L->iso_mass[k] : 1.0;
+        tab.ab[k] = used ? L->iso_ab[k] : 0.0;
+        tab.ss[k] = used ? L->iso_sscale[k] : 0.0;
+        tab.Q[k] = used ? tp->Q[k] : 1.0;
+    }
+    double *gTl = (double *)hk::align_up((size_t)ws, 256);
+    double *gPl = gTl + L->nlines;
+    const double cutv = cut->cut == 0.0 ? 1.0e30 : cut->cut;
+    hipStream_t s = (hipStream_t)stream;
+    if (L->nlines > 0) {
+        hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)((L->nlines + 3) / 4)), dim3(256), 0, s,
+                           L->nu, L->S, L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, L->nlines,
+                           tp->T, tp->P, cut->cut_mode, cutv, g->numin, g->dnu, g->nx, gK, gS, gvy,
+                           gTl, gPl);
+        HK_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(sum2_kernel, dim3(1), dim3(1024), 0, s, gTl, gPl, L->nlines, gTP);
+    HK_LAUNCH_CHECK();
+    return HK_OK;
+}

@@ -1,0 +1,239 @@
+"""Python host side over the C ABI, with torch tensors as device memory.
+
+torch provides the device allocations and the stream; all computation runs in
+the HIP kernels of libheliosk.so.  The functions mirror the reference's
+stages: line parameters (S2_kernel), the Voigt sum (Line6A/B/C_Call), the
+k-table (heliosk.cu:2046-2193) and the adjoint.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (HK_VOIGT_FP32C, HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineState, Lines,
+                   Spectrum, TP, Tune, check, lib)
+
+NA = 6.0221412927e23          # define.h:42
+
+__all__ = ["Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "ktable",
+           "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise _lib.HKError("no GPU visible: the heliosk engine runs only on the HIP device")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def make_grid(numin, numax, dnu):
+    """Grid of the reference (heliosk.cu:448): Nx = (int)((numax-numin)/dnu + 0.5)."""
+    nx = int((numax - numin) / dnu + 0.5)
+    return Grid(float(numin), float(dnu), nx)
+
+
+class Species:
+    """Raw line list of one species on the device, sorted by nu (stable)."""
+
+    def __init__(self, nu, S, EL, vy, n, A=None, delta=None, iso=None, iso_mass=(18.0 / NA,),
+                 iso_ab=(1.0,), iso_sscale=(1.0,), name="species"):
+        nu = np.asarray(nu, dtype=np.float64)
+        order = np.argsort(nu, kind="stable")
+        dev = _dev()
+        nl = nu.size
+
+        def up(a, fill=0.0):
+            a = np.full(nl, fill) if a is None else np.asarray(a, dtype=np.float64)
+            return torch.from_numpy(np.ascontiguousarray(a[order])).to(dev)
+
+        self.name = name
+        self.order = order
+        self.nu = up(nu); self.S = up(S); self.EL = up(EL); self.vy = up(vy); self.n = up(n)
+        self.A = up(A); self.delta = up(delta)
+        self.iso = None if iso is None else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(iso, dtype=np.int32)[order])).to(dev)
+        self.iso_mass = list(iso_mass); self.iso_ab = list(iso_ab)
+        self.iso_sscale = list(iso_sscale)
+        self.max_abs_delta = float(np.max(np.abs(delta))) if delta is not None and nl else 0.0
+        self.nlines = nl
+
+    def lines(self):
+        L = Lines()
+        L.nlines = self.nlines
+        L.nu = self.nu.data_ptr(); L.S = self.S.data_ptr(); L.EL = self.EL.data_ptr()
+        L.A = self.A.data_ptr(); L.delta = self.delta.data_ptr(); L.vy = self.vy.data_ptr()
+        L.n = self.n.data_ptr()
+        L.iso = self.iso.data_ptr() if self.iso is not None else None
+        L.niso = len(self.iso_mass)
+        for k in range(L.niso):
+            L.iso_mass[k] = self.iso_mass[k]; L.iso_ab[k] = self.iso_ab[k]
+            L.iso_sscale[k] = self.iso_sscale[k]
+        L.max_abs_delta = self.max_abs_delta
+        return L
+
+
+class State:
+    """Line state at one (T, P): one device allocation laid out by the library."""
+
+    def __init__(self, nlines):
+        nbytes = lib().hk_line_state_bytes(nlines)
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+        self.st = LineState()
+        check(lib().hk_line_state_init(ctypes.byref(self.st), nlines, _ptr(self.buf), nbytes))
+        self.nlines = nlines
+
+    def array(self, field):
+        """Device view of one per-line field as a torch tensor (copy)."""
+        addr = getattr(self.st, field)
+        off = addr - self.buf.data_ptr()
+        return self.buf[off:off + 8 * self.nlines].view(torch.float64).clone()
+
+
+def _tp(T, P, Q):
+    tp = TP()
+    tp.T = float(T); tp.P = float(P)
+    Q = [1.0] if Q is None else list(Q)
+    for k in range(HK_MAX_ISO):
+        tp.Q[k] = Q[k] if k < len(Q) else 1.0
+    return tp
+
+
+def line_params(species, T, P, Q=None, cut_mode=0, cut=0.0, state=None, stream=None):
+    """S2_kernel semantics at (T [K], P [atm]); returns a State."""
+    st = state or State(species.nlines)
+    L = species.lines()
+    tp = _tp(T, P, Q)
+    c = Cut(int(cut_mode), float(cut))
+    check(lib().hk_line_params(ctypes.byref(L), ctypes.byref(tp), ctypes.byref(c),
+                               ctypes.byref(st.st), _stream(stream)))
+    st.species = species        # st.st.key points into species.nu: keep it alive
+    return st
+
+
+def _tune(tile_points=0):
+    return Tune(int(tile_points), 0)
+
+
+def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, accumulate=False,
+          stream=None, workspace=None):
+    """K_j = kmin + sum_i S_i Re w / sqrt(pi) on the grid (or K += sum)."""
+    tune = _tune(tile_points)
+    if K is None:
+        K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
+    nbytes = lib().hk_voigt_workspace_bytes(ctypes.byref(grid), ctypes.byref(tune))
+    ws = workspace if workspace is not None else torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+    check(lib().hk_voigt_accumulate(ctypes.byref(state.st), ctypes.byref(grid), float(kmin),
+                                    int(bool(accumulate)), int(mode), ctypes.byref(tune),
+                                    _ptr(K), _ptr(ws), ws.numel(), _stream(stream)))
+    return K
+
+
+def pair_count(state, grid, stream=None):
+    """(evaluations, region A, region B, region C) of one Voigt sum."""
+    out = torch.zeros(4, dtype=torch.int64, device=_dev())
+    check(lib().hk_voigt_pair_count(ctypes.byref(state.st), ctypes.byref(grid), _ptr(out),
+                                    _stream(stream)))
+    return [int(v) for v in out.cpu()]
+
+
+def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
+    """items: list of (State, Grid, kmin, K tensor).  One persistent launch."""
+    n = len(items)
+    arr = (Spectrum * n)()
+    for i, (st, g, kmin, K) in enumerate(items):
+        arr[i].st = st.st; arr[i].grid = g; arr[i].kmin = float(kmin)
+        arr[i].accumulate = 0; arr[i].K = K.data_ptr()
+    tune = _tune(tile_points)
+    nbytes = lib().hk_voigt_batched_workspace_bytes(n, arr, ctypes.byref(tune))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+    check(lib().hk_voigt_batched(n, arr, int(mode), ctypes.byref(tune), _ptr(ws), nbytes,
+                                 _stream(stream)))
+
+
+class KTable:
+    """Prepared k-table workspace for one (grid, numax, nbins, nC, nspec)."""
+
+    def __init__(self, grid, numax, nbins, nC, nspec=1):
+        self.grid, self.numax, self.nbins, self.nC, self.nspec = grid, float(numax), nbins, nC, nspec
+        nbytes = lib().hk_ktable_workspace_bytes(grid.nx, nbins, nC, nspec)
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+        check(lib().hk_ktable_prepare(ctypes.byref(grid), self.numax, nbins, nC, _ptr(self.ws),
+                                      nbytes))
+        self.Nxb = grid.nx // nbins
+
+    def __call__(self, K, kmin=0.0, unit_scale=1.0, want_sorted=False, stream=None):
+        nspec = K.numel() // self.grid.nx
+        assert nspec <= self.nspec
+        dev = K.device
+        coeff = torch.empty(nspec * self.nbins * self.nC, dtype=torch.float64, device=dev)
+        nxmin = torch.empty(nspec * self.nbins, dtype=torch.int32, device=dev)
+        Ks = torch.empty_like(K) if want_sorted else None
+        check(lib().hk_ktable(_ptr(K), nspec, ctypes.byref(self.grid), self.numax, self.nbins,
+                              self.nC, float(kmin), float(unit_scale), _ptr(coeff), _ptr(nxmin),
+                              _ptr(Ks), _ptr(self.ws), self.ws.numel(), _stream(stream)))
+        coeff = coeff.view(nspec, self.nbins, self.nC)
+        nxmin = nxmin.view(nspec, self.nbins)
+        return (coeff, nxmin, Ks) if want_sorted else (coeff, nxmin)
+
+
+def ktable(K, grid, numax, nbins, nC, kmin=0.0, unit_scale=1.0, want_sorted=False):
+    kt = KTable(grid, numax, nbins, nC, nspec=max(1, K.numel() // grid.nx))
+    return kt(K, kmin, unit_scale, want_sorted)
+
+
+def ktable_reconstruct(coeff, nxmin, Nxb, kmin=0.0, unit_scale=1.0, stream=None):
+    nspec, nbins, nC = coeff.shape
+    Ky = torch.empty(nspec * nbins * Nxb, dtype=torch.float64, device=coeff.device)
+    scratch = torch.empty_like(Ky)
+    check(lib().hk_ktable_reconstruct(_ptr(coeff.contiguous()), _ptr(nxmin.contiguous()), nspec,
+                                      nbins, Nxb, nC, float(kmin), float(unit_scale), _ptr(Ky),
+                                      _ptr(scratch), _stream(stream)))
+    return Ky
+
+
+def ktable_plan(Nxb, nC):
+    P = np.empty(nC * 2 * nC)
+    check(lib().hk_ktable_plan_host(int(Nxb), int(nC), P.ctypes.data_as(_lib.P_d)))
+    return P.reshape(nC, 2 * nC)
+
+
+def adjoint(species, T, P, gK, grid, Q=None, cut_mode=0, cut=0.0, stream=None):
+    """(dL/dT, dL/dP, dL/dS_i, dL/dvy_i) for L = sum_j gK_j K_j, in the
+    species' sorted line order."""
+    L = species.lines()
+    tp = _tp(T, P, Q)
+    c = Cut(int(cut_mode), float(cut))
+    dev = gK.device
+    gS = torch.empty(species.nlines, dtype=torch.float64, device=dev)
+    gv = torch.empty_like(gS)
+    gTP = torch.empty(2, dtype=torch.float64, device=dev)
+    nbytes = lib().hk_voigt_adjoint_workspace_bytes(species.nlines)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    check(lib().hk_voigt_adjoint(ctypes.byref(L), ctypes.byref(tp), ctypes.byref(c),
+                                 ctypes.byref(grid), _ptr(gK.contiguous()), _ptr(gS), _ptr(gv),
+                                 _ptr(gTP), _ptr(ws), nbytes, _stream(stream)))
+    return gTP, gS, gv
+
+
+def bin_segments(grid, numax, nbins):
+    seg = (ctypes.c_int64 * (nbins + 2))()
+    check(lib().hk_bin_segments(ctypes.byref(grid), float(numax), int(nbins), seg))
+    return np.array(seg[:], dtype=np.int64)
+
+
+def synth_lines(seed, species, n, kind=0):
+    """Seeded synthetic line list (unsorted numpy arrays)."""
+    out = {k: np.empty(n) for k in ("nu", "S", "EL", "gamma", "n")}
+    P = _lib.P_d
+    check(lib().hk_synth_lines(ctypes.c_uint64(seed), int(species), int(n), int(kind),
+                               *[out[k].ctypes.data_as(P) for k in ("nu", "S", "EL", "gamma", "n")]))
+    return out

@@ -1,0 +1,211 @@
+/*
+ * heliosk.h -- C ABI of the MI355X line-by-line opacity engine.
+ *
+ * Plain C, plain pointers and sizes.  Every device buffer is owned by the
+ * caller (as every device buffer of the reference is, host.h:792-876,
+ * heliosk.cu:712-725); entry points that need scratch memory take a
+ * workspace whose size a *_workspace_bytes() query returns.  Every call
+ * takes an explicit stream (a hipStream_t; NULL = the default stream) and
+ * returns HK_OK (0) or an error code; hk_last_error() gives the text of the
+ * calling thread's last error.
+ *
+ * Reference seams replaced (exoclime/HELIOS-K 2.08):
+ *   hk_line_params       S2_kernel (voigt.h:331-363) + the per-line part of
+ *                        readFile (host.h:641-644) + the cut of S3_kernel
+ *                        (voigt.h:436-482)
+ *   hk_voigt_accumulate  Line6A_Call/Line6B_Call/Line6C_Call (voigt.h:1820,
+ *                        1968, 2004) with nuLimits/iiLimits (voigt.h:2045-2341)
+ *                        and AddKStreams_kernel (voigt.h:728)
+ *   hk_voigt_batched     the species/file/pressure loops of main
+ *                        (heliosk.cu:784-1790) around those calls
+ *   hk_ktable            the per-bin sort and resampling sequence of main
+ *                        (heliosk.cu:2046-2193: thrust sorts, findCut,
+ *                        rescale, copyK2, lnK, Vandermonde/QR, leastSquare)
+ *   hk_ktable_reconstruct expfx_kernel + rescale(f=-1) (heliosk.cu:2189-2193)
+ *   hk_voigt_adjoint     no reference counterpart (gradient of the binned
+ *                        opacity; see the function comment)
+ */
+#ifndef HELIOSK_H
+#define HELIOSK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *hk_stream_t;      /* hipStream_t */
+
+enum {
+    HK_OK = 0,
+    HK_ERR_ARG = 1,             /* invalid argument (message says which) */
+    HK_ERR_HIP = 2,             /* HIP runtime error */
+    HK_ERR_WORKSPACE = 3,       /* workspace too small */
+    HK_ERR_UNSUPPORTED = 4
+};
+
+#define HK_MAX_ISO 16
+
+/* Voigt evaluation forms. */
+enum {
+    HK_VOIGT_FP64 = 0,          /* plain FP64 profile, FP64 accumulation */
+    HK_VOIGT_FP32C = 1          /* FP32 profile, compensated FP32 tile sums */
+};
+
+/* Raw line list of one species, device SoA, sorted by nu ascending.
+ * Per line: nu [cm^-1], S = S_file (the strength as stored in the line
+ * file), EL [cm^-1], A [s^-1], delta [cm^-1/atm], vy = ((1-q) g_air +
+ * q g_self) * gammaF [cm^-1/atm], n (temperature exponent), iso (index
+ * into the isotopologue tables; NULL means 0 for every line).
+ * Strength at (T,P) follows readFile (host.h:641-644) and S2_kernel:
+ * S_h = S_file / Q[iso] * iso_ab[iso] * iso_sscale[iso]. */
+typedef struct {
+    int64_t nlines;
+    const double *nu, *S, *EL, *A, *delta, *vy, *n;
+    const int32_t *iso;
+    int niso;
+    double iso_mass[HK_MAX_ISO];    /* mass of one molecule [g] (M / N_A) */
+    double iso_ab[HK_MAX_ISO];
+    double iso_sscale[HK_MAX_ISO];
+    double max_abs_delta;           /* max |delta| over the lines */
+} hk_lines;
+
+/* One thermodynamic point. */
+typedef struct {
+    double T;                       /* K */
+    double P;                       /* atm */
+    double Q[HK_MAX_ISO];           /* partition function at T, per isotopologue */
+} hk_tp;
+
+/* Line cut: cut_mode 0 = cut in cm^-1, 1 = in Lorentz half widths,
+ * 2 = in Doppler widths (S3_kernel, voigt.h:442-450).  cut == 0 means
+ * no cut (heliosk.cu:941). */
+typedef struct {
+    int cut_mode;
+    double cut;
+} hk_cut;
+
+/* Uniform grid: x_j = fma(j, dnu, numin), j = 0 .. nx-1 (setX_kernel). */
+typedef struct {
+    double numin, dnu;
+    int64_t nx;
+} hk_grid;
+
+/* Line state at one (T,P), written by hk_line_params.  Device SoA. */
+typedef struct {
+    int64_t nlines;
+    double *nu;         /* shifted centre nu + delta P */
+    double *S;          /* S(T) * (1/alpha_D), as S2_kernel leaves it */
+    double *ialphaD;    /* 1 / alpha_D */
+    double *y;          /* alpha_L / alpha_D */
+    double *hw;         /* cut half width [cm^-1]; < 0 marks an empty line */
+    double *blk_max;    /* max S over each block of 256 lines, ceil(n/256) */
+    double *radius;     /* 1 value: max_i (hw_i + |nu_i - key_i|) */
+    const double *key;  /* sorted search key (the raw nu), set by hk_line_params */
+} hk_line_state;
+
+/* Launch tuning; zero fields select the defaults. */
+typedef struct {
+    int tile_points;    /* grid points per workgroup tile */
+    int lines_per_stage;/* lines staged through LDS per step */
+} hk_tune;
+
+const char *hk_last_error(void);
+const char *hk_version(void);
+
+/* Bytes of device memory an hk_line_state of n lines needs (one
+ * allocation holding nu, S, ialphaD, y, hw, blk_max and radius). */
+size_t hk_line_state_bytes(int64_t nlines);
+/* Lay out an hk_line_state inside caller memory of hk_line_state_bytes. */
+int hk_line_state_init(hk_line_state *st, int64_t nlines, void *mem, size_t bytes);
+
+/* Line parameters at one (T,P) -- S2_kernel semantics, FP64. */
+int hk_line_params(const hk_lines *lines, const hk_tp *tp, const hk_cut *cut,
+                   hk_line_state *st, hk_stream_t stream);
+
+/* Voigt sum onto the grid: K_j = kmin + sum_i S_i Re w(x_ij, y_i)/sqrt(pi)
+ * (accumulate == 0) or K_j += sum (accumulate != 0), over the lines whose
+ * cut window holds j: |x_j - nu_i| < hw_i in FP64.  Deterministic: each
+ * point is owned by one thread that adds the lines in index order. */
+size_t hk_voigt_workspace_bytes(const hk_grid *grid, const hk_tune *tune);
+int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
+                        double kmin, int accumulate, int mode,
+                        const hk_tune *tune, double *K,
+                        void *workspace, size_t workspace_bytes,
+                        hk_stream_t stream);
+
+/* Algorithmic work of one Voigt sum: counts[0] = line-point pairs inside
+ * the cut windows (the evaluations), counts[1..3] = those with
+ * x^2+y^2 >= 1e6, in [100, 1e6), < 100 (the reference's regions A, B, C,
+ * voigt.h:1223-1230, 1490, 1665).  counts: 4 device uint64. */
+int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
+                        unsigned long long *counts, hk_stream_t stream);
+
+/* Many spectra in one persistent launch (work queue over spectrum tiles). */
+typedef struct {
+    hk_line_state st;
+    hk_grid grid;
+    double kmin;
+    int accumulate;
+    double *K;          /* device, grid.nx values */
+} hk_spectrum;
+
+size_t hk_voigt_batched_workspace_bytes(int nspec, const hk_spectrum *spec,
+                                        const hk_tune *tune);
+int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode,
+                     const hk_tune *tune, void *workspace,
+                     size_t workspace_bytes, hk_stream_t stream);
+
+/* k-distribution tables.  K holds nspec spectra of grid->nx points; bins are
+ * equal in wavenumber over [numin, numax) and hold Nxb = nx/nbins points.
+ * hk_ktable_prepare fills the workspace with the per-(Nxb, nC) least-squares
+ * map once; hk_ktable then sorts each bin ascending (Ksorted, which may be
+ * NULL), finds Nxmin, rescales, takes ln(K * unit_scale) and fits nC
+ * Chebyshev coefficients per bin (coeff[(s*nbins + b)*nC + c]). */
+size_t hk_ktable_workspace_bytes(int64_t nx, int nbins, int nC, int nspec);
+int hk_ktable_prepare(const hk_grid *grid, double numax, int nbins, int nC,
+                      void *workspace, size_t workspace_bytes);
+int hk_ktable(const double *K, int nspec, const hk_grid *grid, double numax,
+              int nbins, int nC, double kmin, double unit_scale,
+              double *coeff, int32_t *nxmin, double *Ksorted,
+              void *workspace, size_t workspace_bytes, hk_stream_t stream);
+/* K(y) from the coefficients: exp(Chebyshev sum) on y = k/Nxb, mapped back
+ * over [Nxmin, Nxb) (expfx_kernel + rescale f=-1).  scratch: nspec*nbins*Nxb
+ * doubles of device memory. */
+int hk_ktable_reconstruct(const double *coeff, const int32_t *nxmin, int nspec,
+                          int nbins, int Nxb, int nC, double kmin,
+                          double unit_scale, double *Ky, double *scratch,
+                          hk_stream_t stream);
+/* The least-squares map of hk_ktable_prepare on the host: P is nC x 2nC,
+ * c = P[:, :nC] m + P[:, nC:] b0 with m_i = sum_k T_i(x_k) b_k and
+ * b0 = b[0:nC]; it reproduces leastSquare_kernel (NC-1 reflections). */
+int hk_ktable_plan_host(int Nxb, int nC, double *P);
+
+/* Adjoint of the Voigt sum: given gK_j = dL/dK_j on the grid, writes
+ * gS[i] = dL/dS_i (S = the raw S_file), gvy[i] = dL/dvy_i and
+ * gTP[0..1] = dL/dT, dL/dP, holding the partition function fixed.  The
+ * profile is recomputed (FP64), not stored.  gTP is device memory. */
+size_t hk_voigt_adjoint_workspace_bytes(int64_t nlines);
+int hk_voigt_adjoint(const hk_lines *lines, const hk_tp *tp, const hk_cut *cut,
+                     const hk_grid *grid, const double *gK, double *gS,
+                     double *gvy, double *gTP, void *workspace,
+                     size_t workspace_bytes, hk_stream_t stream);
+
+/* Host helpers. */
+/* First grid index of each bin by binKey semantics (voigt.h:779-806):
+ * seg[b], b = 0..nbins+1 (seg[nbins+1] = nx). */
+int hk_bin_segments(const hk_grid *grid, double numax, int nbins, int64_t *seg);
+/* Seeded synthetic line list (counter-based SplitMix64; identical on
+ * every host): fields nu, S, EL, gamma (= g_air = g_self), n.  kind 0:
+ * nu U[0,30000), log10 S U[-30,-19], EL U[0,20000), gamma U[0.01,0.1],
+ * n U[0.3,0.8]; kind 1: gamma log-uniform [1e-4, 1] and 5% strong lines
+ * with log10 S U[-20,-17].  Unsorted. */
+int hk_synth_lines(uint64_t seed, int species, int64_t n, int kind,
+                   double *nu, double *S, double *EL, double *gamma,
+                   double *nexp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -505,6 +505,35 @@ void hko_ktable(double *K, int64_t Nx, int Nxb, int nbins, int nC,
 }
 
 /* ------------------------------------------------------------------ */
+/* leastSquare_kernel alone (resample.h:243-353) on an arbitrary b of  */
+/* length Nxb: NC-1 reflections then back substitution; c[0..nC).      */
+/* ------------------------------------------------------------------ */
+void hko_lsq(int Nxb, int nC, const double *b_in, double *c)
+{
+    double *V = (double *)malloc((size_t)Nxb * nC * sizeof(double));
+    double *C = (double *)malloc(nC * sizeof(double));
+    double *D = (double *)malloc(nC * sizeof(double));
+    double *bb = (double *)malloc((size_t)Nxb * sizeof(double));
+    hko_qr(Nxb, nC, V, C, D);
+    memcpy(bb, b_in, (size_t)Nxb * sizeof(double));
+    for (int j = 0; j < nC - 1; ++j) {
+        const double *vj = V + (size_t)Nxb * j;
+        double dot = 0.0;
+        for (int k = j; k < Nxb; ++k) dot += vj[k] * bb[k];
+        double t = dot / C[j];
+        for (int k = j; k < Nxb; ++k) bb[k] -= t * vj[k];
+    }
+    bb[nC - 1] /= D[nC - 1];
+    for (int i = nC - 2; i >= 0; --i) {
+        double a = 0.0;
+        for (int j = i + 1; j < nC; ++j) a += V[i + (size_t)Nxb * j] * bb[j];
+        bb[i] = (bb[i] - a) / D[i];
+    }
+    memcpy(c, bb, (size_t)nC * sizeof(double));
+    free(V); free(C); free(D); free(bb);
+}
+
+/* ------------------------------------------------------------------ */
 /* Reconstruction K(y): expfx_kernel + rescale(f=-1) (resample.h:497-524, */
 /* 398-437); Ky has nbins*Nxb entries.                                 */
 /* ------------------------------------------------------------------ */

@@ -77,6 +77,7 @@ void hko_sort_segments(double *K, int nseg, const int64_t *seg);
 void hko_qr(int Nxb, int nC, double *V, double *C, double *D);
 void hko_ktable(double *K, int64_t Nx, int Nxb, int nbins, int nC,
                 double kmin, double unitScale, double *coeff, int *Nxmin);
+void hko_lsq(int Nxb, int nC, const double *b, double *c);
 void hko_reconstruct(const double *coeff, const int *Nxmin, int Nxb,
                      int nbins, int nC, double kmin, double unitScale,
                      double *Ky);

@@ -49,6 +49,7 @@ def lib():
         L.hko_sort_segments.argtypes = [_D, c_int, _I64]
         L.hko_qr.argtypes = [c_int, c_int, _D, _D, _D]
         L.hko_ktable.argtypes = [_D, c_i64, c_int, c_int, c_int, c_d, c_d, _D, _INT]
+        L.hko_lsq.argtypes = [c_int, c_int, _D, _D]
         L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
         _lib = L
     return _lib
@@ -175,6 +176,13 @@ def ktable(Ksorted, Nxb, nbins, nC, kmin, unit_scale=1.0):
     lib().hko_ktable(_p(K), K.size, int(Nxb), int(nbins), int(nC), float(kmin),
                      float(unit_scale), _p(coeff), _p(nxmin, _INT))
     return coeff.reshape(nbins, nC), nxmin.astype(np.int64)
+
+
+def lsq(b, nC):
+    """leastSquare_kernel alone on an arbitrary right-hand side b."""
+    b = _f64(b); c = np.empty(nC)
+    lib().hko_lsq(int(b.size), int(nC), _p(b), _p(c))
+    return c
 
 
 def reconstruct(coeff, nxmin, Nxb, kmin, unit_scale=1.0):

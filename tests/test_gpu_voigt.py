@@ -1,0 +1,167 @@
+"""GPU parity of the line-parameter and Voigt kernels against the FP64 oracle.
+
+Run on an MI355X: python -m pytest tests -m gpu.  The oracle is the checker
+only (tests import it; the engine never does).
+"""
+import numpy as np
+import pytest
+
+from helpers import random_lines, rel_err, species_from, state_numpy
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11      # plain FP64 form vs the oracle
+FP32C_TOL = 1e-5      # FP32 compensated form vs the oracle
+
+
+@pytest.fixture(scope="module")
+def E():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from heliosk import engine
+    return engine
+
+
+LP_CASES = [
+    # T, P[atm], cut_mode, cut, delta, A, niso
+    (296.0, 1.0, 0, 25.0, 0.0, 0.0, 1),
+    (1500.0, 0.3, 0, 100.0, -0.003, 0.0, 1),
+    (100.0, 1e-6, 1, 100.0, 0.0, 5e6, 3),
+    (3000.0, 100.0, 2, 50.0, 0.01, 0.0, 2),
+    (800.0, 2.0, 0, 0.0, 0.0, 0.0, 1),
+]
+
+
+@pytest.mark.parametrize("case", LP_CASES)
+def test_line_params_vs_oracle(E, case):
+    T, P, cm, cut, delta, A, niso = case
+    d = random_lines(5000, seed=11, delta=delta, A=A, niso=niso, EL=(0.0, 20000.0))
+    sp = species_from(d)
+    st = E.line_params(sp, T, P, Q=d["iso_Q"], cut_mode=cm, cut=cut)
+    g = state_numpy(st)
+    o = O.line_params(d, T, P, cm, 1e30 if cut == 0.0 else cut)
+    for k in ("nu", "S", "ialphaD", "y"):
+        np.testing.assert_allclose(g[k], o[k], rtol=2e-13, atol=0, err_msg=k)
+    np.testing.assert_allclose(g["hw"], o["hw"], rtol=2e-13, atol=0)
+
+
+def _grid(E, numin, numax, dnu):
+    return E.make_grid(numin, numax, dnu)
+
+
+def _compare(E, d, T, P, grid, cm, cut, kmin=0.0, tile_points=0, Q=None):
+    sp = species_from(d)
+    st = E.line_params(sp, T, P, Q=Q if Q is not None else d["iso_Q"], cut_mode=cm, cut=cut)
+    ost = state_numpy(st)
+    Ko = O.voigt_points(ost, grid.numin, grid.dnu, grid.nx, kmin=kmin)
+    K64 = E.voigt(st, grid, kmin, E.HK_VOIGT_FP64, tile_points).cpu().numpy()
+    K32 = E.voigt(st, grid, kmin, E.HK_VOIGT_FP32C, tile_points).cpu().numpy()
+    return Ko, K64, K32
+
+
+VOIGT_CASES = [
+    # name, T, P, cut_mode, cut, nlines, (numin, numax, dnu), line range
+    ("core-dominated", 1500.0, 1e-3, 0, 2.0, 200, (1000.0, 1010.0, 0.0005), (999.0, 1011.0)),
+    ("pressure-wings", 1000.0, 10.0, 0, 5.0, 300, (1000.0, 1010.0, 0.001), (995.0, 1015.0)),
+    ("lorentz-cut", 296.0, 1.0, 1, 30.0, 300, (1000.0, 1010.0, 0.001), (998.0, 1012.0)),
+    ("doppler-cut", 3000.0, 0.01, 2, 40.0, 300, (1000.0, 1010.0, 0.001), (998.0, 1012.0)),
+    ("no-cut", 700.0, 0.5, 0, 0.0, 60, (1000.0, 1004.0, 0.002), (990.0, 1014.0)),
+    ("low-T-low-P", 100.0, 1e-6, 0, 1.0, 400, (1000.0, 1002.0, 0.0001), (999.5, 1002.5)),
+]
+
+
+@pytest.mark.parametrize("case", VOIGT_CASES, ids=[c[0] for c in VOIGT_CASES])
+def test_voigt_vs_oracle(E, case):
+    name, T, P, cm, cut, n, (a, b, dnu), (la, lb) = case
+    d = random_lines(n, seed=hash(name) % 1000, numin=la, numax=lb, EL=(0.0, 3000.0))
+    grid = _grid(E, a, b, dnu)
+    Ko, K64, K32 = _compare(E, d, T, P, grid, cm, cut, kmin=0.0)
+    e64, e32 = rel_err(K64, Ko), rel_err(K32, Ko)
+    print("%s: FP64 %.3e  FP32C %.3e" % (name, e64, e32))
+    assert e64 <= FP64_TOL
+    assert e32 <= FP32C_TOL
+    # exactly-empty points agree (shared FP64 integer windows)
+    np.testing.assert_array_equal(K64 == 0.0, Ko == 0.0)
+    np.testing.assert_array_equal(K32 == 0.0, Ko == 0.0)
+
+
+def test_voigt_edges_and_kmin(E):
+    # lines outside the grid, at both edges, a zero-width (P=0, A=0) Doppler
+    # line, kmin > 0 and accumulate
+    d = random_lines(50, seed=5, numin=999.0, numax=1001.0)
+    d["nu"] = np.sort(np.r_[d["nu"][:-4], 990.0, 1000.0, 1000.99995, 1020.0])
+    grid = _grid(E, 1000.0, 1001.0, 0.0001)
+    Ko, K64, K32 = _compare(E, d, 500.0, 0.0, grid, 0, 0.5, kmin=1e-30)
+    assert rel_err(K64, Ko) <= FP64_TOL and rel_err(K32, Ko) <= FP32C_TOL
+    sp = species_from(d)
+    st = E.line_params(sp, 500.0, 0.0, Q=d["iso_Q"], cut_mode=0, cut=0.5)
+    K = E.voigt(st, grid, 1e-30, E.HK_VOIGT_FP64)
+    K2 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, K=K.clone() * 0 + 5.0, accumulate=True)
+    np.testing.assert_allclose(K2.cpu().numpy(), 5.0 + (K.cpu().numpy() - 1e-30), rtol=1e-15)
+
+
+def test_voigt_empty_inputs(E):
+    import torch
+    d = random_lines(0, seed=1)
+    grid = _grid(E, 1000.0, 1001.0, 0.001)
+    sp = species_from(d)
+    st = E.line_params(sp, 500.0, 1.0, Q=d["iso_Q"], cut_mode=0, cut=1.0)
+    for mode in (E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C):
+        K = E.voigt(st, grid, 2.5e-7, mode)
+        assert torch.all(K == 2.5e-7)
+
+
+def test_voigt_tiles_and_determinism(E):
+    d = random_lines(500, seed=9, numin=995.0, numax=1015.0)
+    grid = _grid(E, 1000.0, 1010.0, 0.001)
+    sp = species_from(d)
+    st = E.line_params(sp, 900.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=5.0)
+    ref = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, 1024).cpu().numpy()
+    # FP64: every point adds its lines in index order whatever the tile
+    for tp in (512, 2048, 4096):
+        np.testing.assert_array_equal(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, tp).cpu().numpy(), ref)
+    r32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, 2048).cpu().numpy()
+    np.testing.assert_array_equal(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, 2048).cpu().numpy(), r32)
+    for tp in (512, 1024, 4096):
+        assert rel_err(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, tp).cpu().numpy(), ref) <= FP32C_TOL
+
+
+def test_voigt_batched_matches_single(E):
+    import torch
+    items, refs = [], []
+    grid = _grid(E, 1000.0, 1010.0, 0.001)
+    for s, n in enumerate((40, 400, 1200)):
+        d = random_lines(n, seed=20 + s, numin=990.0, numax=1020.0)
+        sp = species_from(d)
+        for T, P in ((300.0, 0.1), (2000.0, 5.0)):
+            st = E.line_params(sp, T, P, Q=d["iso_Q"], cut_mode=0, cut=8.0)
+            for mode in (E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C):
+                refs.append((mode, E.voigt(st, grid, 0.0, mode).cpu().numpy()))
+            items.append(st)
+    for mi, mode in enumerate((E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C)):
+        outs = [torch.empty(grid.nx, dtype=torch.float64, device="cuda") for _ in items]
+        E.voigt_batched([(st, grid, 0.0, K) for st, K in zip(items, outs)], mode=mode)
+        for k, K in enumerate(outs):
+            np.testing.assert_array_equal(K.cpu().numpy(), refs[2 * k + mi][1])
+
+
+def test_config0_fp64_and_fp32c_subsample(E):
+    """Workload c0 (1e5 lines, 3e6 points): both forms vs the oracle on 20000
+    random grid points, and FP32C vs FP64 on the whole grid."""
+    from heliosk import configs
+    w = configs.WORKLOADS["c0"]
+    arr = configs.species_arrays(w, 0)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    T, Pbar = w.tp[0]
+    st = E.line_params(sp, T, Pbar * configs.POBAR, Q=[1.0], cut_mode=0, cut=w.cut)
+    grid = _grid(E, w.numin, w.numax, w.dnu)
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    K32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C).cpu().numpy()
+    jj = np.sort(np.random.default_rng(0).choice(grid.nx, 20000, replace=False))
+    Ko = O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj)
+    e64, e32, e3264 = rel_err(K64[jj], Ko), rel_err(K32[jj], Ko), rel_err(K32, K64)
+    print("c0: FP64 vs oracle %.3e, FP32C vs oracle %.3e, FP32C vs FP64 (full grid) %.3e"
+          % (e64, e32, e3264))
+    assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
