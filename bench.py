@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one device copy of this many MiB first (HBM counter calibration)")
     ap.add_argument("--cpu-lines", type=int, default=10000,
                     help="lines of the CPU-baseline sample (first lines by generator index)")
     args = ap.parse_args()
@@ -66,6 +68,14 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     log("bench: workload %s (%s)" % (args.workload, w.note))
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: copy probe of %d MiB done" % args.copy_probe_mb)
+        del src, dst
     t0 = time.time()
     arr = configs.species_arrays(w, 0)
     sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
@@ -170,6 +180,7 @@ def main():
         "fp64_voigt_evals_per_s": evals / (fp64_ms * 1e-3) if fp64_ms else None,
         "speedup_fp32c_vs_fp64": fp64_ms / voigt_ms if fp64_ms else None,
         "fp32c_vs_fp64_max_rel_err": rel,
+        "precision_ok": (rel is not None and rel <= 1e-5),
         "ktable_ms": ktable_ms,
         "ktable_points_per_s": grid.nx / (ktable_ms * 1e-3),
         "ktable_hbm_GBs_algorithmic": 8.0 * grid.nx / (ktable_ms * 1e-3) / 1e9,

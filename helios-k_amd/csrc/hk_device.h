@@ -68,7 +68,11 @@ __device__ __forceinline__ double rcp64(double x) {
 // symmetric pair folds to 2 w u / D with u = r2 + t^2, D = u^2 - 4 t^2 x^2.
 // --------------------------------------------------------------------------
 constexpr double kRB64 = 100.0, kRM64 = 1.0e4, kRF64 = 1.0e6;
-constexpr float kRB32 = 64.0f, kRF32 = 4096.0f;
+// FP32: core r2 < 25 (Algorithm 916), 25..100 8-node Gauss-Hermite plus the
+// Doppler term e^{-x^2} for near-Doppler lines (y < kY0), 100..4096 4-node
+// Gauss-Hermite, >= 4096 two-term asymptotic.
+constexpr float kRB32 = 25.0f, kRM32 = 100.0f, kRF32 = 4096.0f;
+constexpr float kY0 = 0.1f;
 
 // Hermite nodes/weights (positive half), n = 4 and n = 8.
 constexpr double kGH4t1 = 0.5246476232752904, kGH4t2 = 1.6506801238857842;
@@ -112,73 +116,140 @@ __device__ __forceinline__ float gh4_sum32(float x2, float r2) {
     return num * rcp32(D1 * D2);
 }
 
+__device__ __forceinline__ float gh8_sum32(float x2, float r2) {
+    const float a1 = (float)(kGH8t1 * kGH8t1), a2 = (float)(kGH8t2 * kGH8t2);
+    const float a3 = (float)(kGH8t3 * kGH8t3), a4 = (float)(kGH8t4 * kGH8t4);
+    const float u1 = r2 + a1, u2 = r2 + a2, u3 = r2 + a3, u4 = r2 + a4;
+    const float D1 = fmaf(u1, u1, -4.0f * a1 * x2);
+    const float D2 = fmaf(u2, u2, -4.0f * a2 * x2);
+    const float D3 = fmaf(u3, u3, -4.0f * a3 * x2);
+    const float D4 = fmaf(u4, u4, -4.0f * a4 * x2);
+    const float D12 = D1 * D2, D34 = D3 * D4;
+    const float n12 = fmaf((float)kGH8w1 * u1, D2, (float)kGH8w2 * u2 * D1);
+    const float n34 = fmaf((float)kGH8w3 * u3, D4, (float)kGH8w4 * u4 * D3);
+    return fmaf(n12, D34, n34 * D12) * rcp32(D12 * D34);
+}
+
 // --------------------------------------------------------------------------
 // Algorithm 916 (Zaghloul & Ali, ACM TOMS 38(2), 2011), real part,
 // x >= 0, y >= 0, for the core region.  The reference evaluates the same
-// series for its region C (voigt.h:217-268, 1659-1686).
-// a = pi / sqrt(-ln(tol/2)); the sigma series keep the terms whose
-// exponent is above -NT^2 (relative 1e-9 in FP32, 1e-16 in FP64).
+// series for its region C (voigt.h:217-268, 1659-1686):
+//   Re w = e^{-x^2} erfcx(y) cos 2xy + (2a/pi) [sin^2(xy)/y e^{-x^2}
+//          + y (-cos(2xy) s1 + (s2 + s3)/2)]
+//   s1 = sum_n e^{-(a^2n^2 + x^2)} d_n,  s2 = sum_n e^{-(an+x)^2} d_n,
+//   s3 = sum_n e^{-(an-x)^2} d_n,        d_n = 1/(a^2 n^2 + y^2),
+// a = pi/sqrt(-ln(tol/2)).  y is constant along a line, so erfcx(y) and
+// q1 = sum_n e^{-a^2 n^2} d_n (s1 = e^{-x^2} q1) are computed once per line
+// and tile (core_line_consts*).  The series keep the terms whose exponent is
+// above -(a N)^2: FP64 tol 1e-15, N = 13 (e^-47); FP32 tol 1e-7, N = 6
+// (e^-21).  The FP64 core builds the exponentials of each series from two
+// exp() calls by the ratio recurrences e^{-(a(n+1)+x)^2} = e^{-(an+x)^2}
+// e^{-2ax} e^{-a^2(2n+1)}; the FP32 core calls the hardware exponential per
+// term.
 // --------------------------------------------------------------------------
-__device__ __noinline__ double rew916_64(double x, double y) {
-    constexpr double a = 0.529275382982092;          // tol 1e-15
-    constexpr int N1 = 13;                            // a*N1 > 6.5
-    x = fabs(x);
-    if (x == 0.0) return erfcx(y);
-    double xx = x * x, yy = y * y;
-    double ex2 = exp(-xx);
-    if (y == 0.0) return ex2;
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int n = 1; n <= N1; ++n) {
-        double an = a * n;
-        double inv = 1.0 / (an * an + yy);
-        s1 += exp(-(an * an + xx)) * inv;
-        s2 += exp(-(an + x) * (an + x)) * inv;
+constexpr double kA64 = 0.529275382982092;          // tol 1e-15
+constexpr int kN64 = 13;
+constexpr double kEA64 = 0.755683658862919;         // e^{-a^2}
+constexpr double kE2A64 = 0.5710577922724486;       // e^{-2a^2}
+constexpr double kE3A64 = 0.4315390418866247;       // e^{-3a^2}
+constexpr double kA32 = 0.7662137865078357;         // tol 1e-7
+constexpr int kN32 = 6;
+
+__device__ __forceinline__ double core_q1(double y, double a, int N) {
+    double s = 0.0, yy = y * y;
+    for (int n = 1; n <= N; ++n) {
+        const double an = a * n;
+        s += exp(-an * an) / fma(an, an, yy);
     }
-    int n0 = (int)ceil(x / a);
-    for (int k = -N1; k <= N1; ++k) {
-        int n = n0 + k;
-        if (n < 1) continue;
-        double an = a * n, d = an - x;
-        s3 += exp(-d * d) / (an * an + yy);
-    }
-    double xy = x * y, sxy, cxy;
-    sincos(xy, &sxy, &cxy);
-    double c2 = fma(cxy, cxy, -sxy * sxy);           // cos(2xy)
-    double sxy2_over_y = sxy * x * (xy < 1e-8 ? 1.0 : sxy / xy);
-    constexpr double ca = 2.0 * a / kPi;
-    return ex2 * erfcx(y) * c2 + ca * (sxy2_over_y * ex2 + y * (-c2 * s1 + 0.5 * (s2 + s3)));
+    return s;
 }
 
-__device__ __noinline__ float rew916_32(float x, float y) {
-    constexpr float a = 0.7662137865078357f;          // tol 1e-7
-    constexpr int N1 = 7;                              // a*N1 > 5.3
+__device__ __noinline__ double rew_core64(double x, double y, double ey, double q1) {
+    constexpr double a = kA64;
+    x = fabs(x);
+    if (x == 0.0) return ey;
+    const double xx = x * x, ex2 = exp(-xx);
+    if (y == 0.0) return ex2;
+    const double yy = y * y;
+    const double t = exp(-2.0 * a * x);
+    double tn = ex2 * kEA64 * t, g = kE3A64 * t, s2 = 0.0;
+    for (int n = 1; n <= kN64; ++n) {
+        const double an = a * n;
+        s2 = fma(tn, rcp64(fma(an, an, yy)), s2);
+        tn *= g;
+        g *= kE2A64;
+    }
+    const int n0 = (int)ceil(x / a);
+    const double dl = a * n0 - x;
+    const double E0 = exp(-dl * dl), e2 = exp(-2.0 * a * dl);
+    double s3 = 0.0;
+    {
+        double E = E0, rho = e2 * kEA64;
+        for (int k = 0; k <= kN64; ++k) {
+            const double an = a * (n0 + k);
+            s3 = fma(E, rcp64(fma(an, an, yy)), s3);
+            E *= rho;
+            rho *= kE2A64;
+        }
+    }
+    {
+        double E = E0, sg = kEA64 * rcp64(e2);
+        for (int k = 1; k <= kN64; ++k) {
+            const int n = n0 - k;
+            if (n < 1) break;
+            E *= sg;
+            sg *= kE2A64;
+            const double an = a * n;
+            s3 = fma(E, rcp64(fma(an, an, yy)), s3);
+        }
+    }
+    const double xy = x * y;
+    double sxy, cxy;
+    sincos(xy, &sxy, &cxy);
+    const double c2 = fma(cxy, cxy, -sxy * sxy);
+    const double sxy2_over_y = sxy * x * (xy < 1e-8 ? 1.0 : sxy / xy);
+    constexpr double ca = 2.0 * a / kPi;
+    return ex2 * ey * c2 + ca * (sxy2_over_y * ex2 + y * (-c2 * (ex2 * q1) + 0.5 * (s2 + s3)));
+}
+
+// FP32 core for r2 < 25: n0 = ceil(x/a) <= 7, so every series term uses
+// n <= kNT32; dtab[n] = 1/(a^2 n^2 + y^2) is filled once per line and tile.
+constexpr int kNT32 = 13;
+
+__device__ __forceinline__ float rew_core32(float x, float y, float ey, float q1,
+                                            const float *dtab) {
+    constexpr float a = (float)kA32;
     x = fabsf(x);
-    if (x == 0.0f) return erfcxf(y);
-    float xx = x * x, yy = y * y;
-    float ex2 = expf(-xx);
+    if (x == 0.0f) return ey;
+    const float xx = x * x, ex2 = expf(-xx);
     if (y == 0.0f) return ex2;
-    float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    float s2 = 0.0f, s3 = 0.0f;
 #pragma unroll
-    for (int n = 1; n <= N1; ++n) {
-        float an = a * (float)n;
-        float inv = rcp32(fmaf(an, an, yy));
-        s1 += __expf(-(fmaf(an, an, xx))) * inv;
-        s2 += __expf(-(an + x) * (an + x)) * inv;
+    for (int n = 1; n <= kN32; ++n) {
+        const float e = a * (float)n + x;
+        s2 = fmaf(__expf(-e * e), dtab[n], s2);
     }
-    int n0 = (int)ceilf(x / a);
+    const int n0 = (int)ceilf(x / a);
 #pragma unroll
-    for (int k = -N1; k <= N1; ++k) {
-        int n = n0 + k;
-        float an = a * (float)n, d = an - x;
-        float t = __expf(-d * d) * rcp32(fmaf(an, an, yy));
-        s3 += n >= 1 ? t : 0.0f;
+    for (int k = -kN32; k <= kN32; ++k) {
+        const int n = n0 + k;
+        const float d = a * (float)n - x;
+        const int nn = min(max(n, 1), kNT32);
+        const float tm = __expf(-d * d) * dtab[nn];
+        s3 += n >= 1 ? tm : 0.0f;
     }
-    float xy = x * y, sxy, cxy;
+    const float xy = x * y;
+    float sxy, cxy;
     sincosf(xy, &sxy, &cxy);
-    float c2 = fmaf(cxy, cxy, -sxy * sxy);
-    float sxy2_over_y = sxy * x * (xy < 1e-4f ? 1.0f : sxy / xy);
-    constexpr float ca = (float)(2.0 * 0.7662137865078357 / kPi);
-    return ex2 * erfcxf(y) * c2 + ca * (sxy2_over_y * ex2 + y * fmaf(-c2, s1, 0.5f * (s2 + s3)));
+    const float c2 = fmaf(cxy, cxy, -sxy * sxy);
+    const float sxy2_over_y = sxy * x * (xy < 1e-4f ? 1.0f : sxy / xy);
+    constexpr float ca = (float)(2.0 * kA32 / kPi);
+    return ex2 * ey * c2 + ca * (sxy2_over_y * ex2 + y * fmaf(-c2, ex2 * q1, 0.5f * (s2 + s3)));
+}
+
+// Reference-form core (per-term exponentials), kept for rew64.
+__device__ __noinline__ double rew916_64(double x, double y) {
+    return rew_core64(x, y, erfcx(y), core_q1(y, kA64, kN64));
 }
 
 // Re w for any r2 in FP64 (used by the adjoint and by tests of the pieces).

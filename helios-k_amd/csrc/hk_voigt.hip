@@ -55,14 +55,19 @@ struct VArgs {
     double *K;
 };
 
+// LDS record of one line for one tile.  lo/hi: the line's cut window;
+// [r1lo, r1hi), [r2lo, r2hi), [r3lo, r3hi): conservative tile-relative index
+// intervals outside which x^2 + y^2 is >= R1, R2, R3 (nested; R3 bounds the
+// core).  A wave whose points miss an interval takes the branch-free path of
+// the next region out.
 struct Rec64 {
-    double nu, ia, y2, y, A, B, G;
-    int32_t lo, hi;
+    double nu, ia, y, y2, A, B, G, p2, p3, p4, ey, q1;
+    int32_t lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi;
 };
 
 struct Rec32 {
-    float r, vbh, vbl, y, y2, A, B, G;
-    int32_t d0, lo, hi, pad;
+    float r, vbh, vbl, y, y2, A, B, G, cy, ey, q1, pad2;
+    int32_t d0, lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi, dop, pad0, pad1;
 };
 
 __device__ __forceinline__ int64_t lower_bound_dev(const double *v, int64_t n, double key) {
@@ -116,36 +121,128 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(
     }
 }
 
+// Conservative tile-relative interval of the points with x^2 + y^2 < R, for
+// x_j ~ (j - jc) vb + rr and tile start j0 = jc + d0 ... as index j - j0.
+__device__ __forceinline__ void region_interval(double R, double y2, double rr, double vb,
+                                                double d0, int W, int32_t &lo, int32_t &hi) {
+    if (!(y2 < R)) { lo = W + 1; hi = -1; return; }
+    const double X = sqrt(R - y2);
+    const double mlo = floor((-X - rr) / vb) - 1.0 - d0;
+    const double mhi = ceil((X - rr) / vb) + 2.0 - d0;
+    lo = (int32_t)fmin(fmax(mlo, -1.0), (double)(W + 1));
+    hi = (int32_t)fmin(fmax(mhi, -1.0), (double)(W + 1));
+}
+
+__device__ __forceinline__ bool misses(int32_t lo, int32_t hi, int wlo, int whi) {
+    return hi <= wlo || lo >= whi;
+}
+
+__device__ __noinline__ void core_consts64(double y, double &ey, double &q1) {
+    ey = erfcx(y);
+    q1 = core_q1(y, kA64, kN64);
+}
+__device__ __noinline__ void core_consts32(float y, float &ey, float &q1) {
+    ey = (float)erfcx((double)y);
+    q1 = (float)core_q1((double)y, kA32, kN32);
+}
+
+// Paths of the per-(line, wave) dispatch.
+enum { kPathFar = 0, kPathMid = 1, kPathMid2 = 2, kPathAny = 3 };
+
 // ---------------------------------------------------------------------------
-// FP64 form
+// FP64 form.  Wave w owns the contiguous points [w*64*PPT, (w+1)*64*PPT) of
+// the tile; lane l holds points w*64*PPT + 64k + l.  FULL: the line's cut
+// window covers the wave's points, so no per-point mask.
 // ---------------------------------------------------------------------------
+template <int PATH, bool FULL, int PPT>
+__device__ __forceinline__ void eval64(const Rec64 &qr, const double *xj, double *acc, int jr0) {
+    const double nu = qr.nu, ia = qr.ia, y2 = qr.y2, B = qr.B, G = qr.G;
+    const double p2 = qr.p2, p3 = qr.p3, p4 = qr.p4;
+    const int32_t lo = qr.lo, hi = qr.hi;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const double x = (xj[k] - nu) * ia;
+        const double r2 = fma(x, x, y2);
+        double v;
+        if (PATH == kPathFar) {
+            const double inv = rcp64(r2);
+            v = B * inv * fma(fma(fma(fma(p4, inv, p3), inv, p2), inv, 1.5), inv, 1.0);
+        } else if (PATH == kPathMid) {
+            v = G * gh4_sum64(x * x, r2);
+        } else if (PATH == kPathMid2) {
+            v = G * gh8_sum64(x * x, r2);
+        } else {
+            if (r2 >= kRF64) {
+                const double inv = rcp64(r2);
+                v = B * inv * fma(fma(fma(fma(p4, inv, p3), inv, p2), inv, 1.5), inv, 1.0);
+            } else if (r2 >= kRM64) {
+                v = G * gh4_sum64(x * x, r2);
+            } else if (r2 >= kRB64) {
+                v = G * gh8_sum64(x * x, r2);
+            } else {
+                v = qr.A * rew_core64(x, qr.y, qr.ey, qr.q1);
+            }
+        }
+        if (!FULL) {
+            const int jr = jr0 + 64 * k;
+            v = (jr >= lo && jr < hi) ? v : 0.0;
+        }
+        acc[k] += v;
+    }
+}
+
+template <bool FULL, int PPT>
+__device__ __forceinline__ void dispatch64(const Rec64 &q, const double *xj, double *acc, int jr0,
+                                           int wlo, int whi) {
+    if (misses(q.r1lo, q.r1hi, wlo, whi)) eval64<kPathFar, FULL, PPT>(q, xj, acc, jr0);
+    else if (misses(q.r2lo, q.r2hi, wlo, whi)) eval64<kPathMid, FULL, PPT>(q, xj, acc, jr0);
+    else if (misses(q.r3lo, q.r3hi, wlo, whi)) eval64<kPathMid2, FULL, PPT>(q, xj, acc, jr0);
+    else eval64<kPathAny, FULL, PPT>(q, xj, acc, jr0);
+}
+
 template <int PPT>
 __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *rec) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wlo = wave * 64 * PPT, whi = wlo + 64 * PPT;
+    const int jr0 = wlo + lane;
     const TileInfo ti = a.tiles[tile];
     const int64_t j0 = tile * W;
     double xj[PPT], acc[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        xj[k] = fma((double)(j0 + t + kThreads * k), a.dnu, a.numin);
+        xj[k] = fma((double)(j0 + jr0 + 64 * k), a.dnu, a.numin);
         acc[k] = 0.0;
     }
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec64 r;
         r.lo = 0; r.hi = 0;
-        r.nu = 0.0; r.ia = 0.0; r.y = 0.0; r.y2 = 0.0; r.A = 0.0; r.B = 0.0; r.G = 0.0;
         if (i < ti.hi) {
             const double nu = a.nu[i];
-            int64_t wlo, whi;
-            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wlo, whi);
-            const int64_t clo = max(wlo, j0), chi = min(whi, j0 + W);
+            int64_t wl, wh;
+            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
+            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
             if (clo < chi) {
-                const double y = a.y[i], A = a.S[i] * kISqrtPi;
-                r.nu = nu; r.ia = a.ia[i]; r.y = y; r.y2 = y * y;
+                const double ia = a.ia[i], y = a.y[i], y2 = y * y;
+                const double A = a.S[i] * kISqrtPi;
+                r.nu = nu; r.ia = ia; r.y = y; r.y2 = y2;
                 r.A = A; r.B = A * y * kISqrtPi; r.G = A * y * (2.0 / kPi);
+                r.p2 = 3.75 - 2.0 * y2; r.p3 = -15.0 * y2; r.p4 = 12.0 * y2 * y2;
                 r.lo = (int32_t)(clo - j0); r.hi = (int32_t)(chi - j0);
+                const double jcd = rint((nu - a.numin) / a.dnu);
+                const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
+                const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
+                region_interval(kRF64, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
+                region_interval(kRM64, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
+                region_interval(kRB64, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
+                if (r.r3lo < r.r3hi && r.r3hi > 0 && r.r3lo < W) {
+                    core_consts64(y, r.ey, r.q1);
+                } else {
+                    r.ey = 0.0; r.q1 = 0.0;
+                }
             }
         }
         __syncthreads();
@@ -153,41 +250,22 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
         __syncthreads();
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
-            const Rec64 q = rec[l];
-            if (q.lo >= q.hi) continue;
-            const bool full = q.lo == 0 && q.hi == W;
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const double x = (xj[k] - q.nu) * q.ia;
-                const double x2 = x * x, r2 = x2 + q.y2;
-                const double inv = rcp64(r2), qq = inv * inv;
-                const double c = 1.0 + fma(3.0, x2, -q.y2) * 0.5 * qq
-                               + 0.75 * fma(fma(5.0, x2, -10.0 * q.y2), x2, q.y2 * q.y2) * qq * qq;
-                double v = q.B * inv * c;
-                if (__any(r2 < kRF64)) {
-                    if (r2 < kRF64) {
-                        if (r2 >= kRM64) v = q.G * gh4_sum64(x2, r2);
-                        else if (r2 >= kRB64) v = q.G * gh8_sum64(x2, r2);
-                        else v = q.A * rew916_64(x, q.y);
-                    }
-                }
-                if (!full) {
-                    const int jr = t + kThreads * k;
-                    v = (jr >= q.lo && jr < q.hi) ? v : 0.0;
-                }
-                acc[k] += v;
-            }
+            const Rec64 &q = rec[l];
+            const int32_t qlo = q.lo, qhi = q.hi;
+            if (qhi <= wlo || qlo >= whi) continue;          // window misses this wave
+            if (qlo <= wlo && qhi >= whi) dispatch64<true, PPT>(q, xj, acc, jr0, wlo, whi);
+            else dispatch64<false, PPT>(q, xj, acc, jr0, wlo, whi);
         }
     }
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const int64_t j = j0 + t + kThreads * k;
+        const int64_t j = j0 + jr0 + 64 * k;
         if (j < a.nx) a.K[j] = a.accumulate ? a.K[j] + acc[k] : a.kmin + acc[k];
     }
 }
 
 // ---------------------------------------------------------------------------
-// FP32 compensated form
+// FP32 compensated form (same point ownership as the FP64 form)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
     const float s = hi + p;
@@ -197,10 +275,72 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
     hi = s;
 }
 
+constexpr int kDT = 16;      // per-line core table stride (entries 1..kNT32 used)
+
+template <int PATH, bool FULL, int PPT>
+__device__ __forceinline__ void eval32(const Rec32 &qr, const float *dtab, float *part, int jr0) {
+    const float r0 = qr.r, vbh = qr.vbh, vbl = qr.vbl, y2 = qr.y2, B = qr.B, G = qr.G, cy = qr.cy;
+    const int32_t lo = qr.lo, hi = qr.hi;
+    const float m0 = (float)(jr0 + qr.d0);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const float m = m0 + (float)(64 * k);
+        const float x = fmaf(m, vbh, fmaf(m, vbl, r0));
+        const float r2 = fmaf(x, x, y2);
+        float v;
+        if (PATH == kPathFar) {
+            const float inv = rcp32(r2);
+            v = (B * inv) * fmaf(fmaf(cy, inv, 1.5f), inv, 1.0f);
+        } else if (PATH == kPathMid) {
+            v = G * gh4_sum32(x * x, r2);
+        } else {
+            if (r2 >= kRF32) {
+                const float inv = rcp32(r2);
+                v = (B * inv) * fmaf(fmaf(cy, inv, 1.5f), inv, 1.0f);
+            } else if (r2 >= kRM32) {
+                v = G * gh4_sum32(x * x, r2);
+            } else if (PATH == kPathMid2 || r2 >= kRB32) {
+                v = G * gh8_sum32(x * x, r2);
+                if (qr.dop) v = fmaf(qr.A, __expf(-x * x), v);
+            } else {
+                v = qr.A * rew_core32(x, qr.y, qr.ey, qr.q1, dtab);
+            }
+        }
+        if (!FULL) {
+            const int jr = jr0 + 64 * k;
+            v = (jr >= lo && jr < hi) ? v : 0.0f;
+        }
+        part[k] += v;
+    }
+}
+
+template <bool FULL, int PPT>
+__device__ __forceinline__ void dispatch32(const Rec32 &q, const float *dtab, float *part, int jr0,
+                                           int wlo, int whi) {
+    if (misses(q.r1lo, q.r1hi, wlo, whi)) {
+        eval32<kPathFar, FULL, PPT>(q, dtab, part, jr0);
+    } else if (misses(q.r2lo, q.r2hi, wlo, whi)) {
+        // 4-node rule for the whole wave unless x is so large that its
+        // single-reciprocal form could overflow FP32
+        const float m0 = (float)(wlo + q.d0), m1 = (float)(whi - 1 + q.d0);
+        const float x0 = fmaf(m0, q.vbh, q.r), x1 = fmaf(m1, q.vbh, q.r);
+        if (fmaxf(x0 * x0, x1 * x1) < 1.0e9f) eval32<kPathMid, FULL, PPT>(q, dtab, part, jr0);
+        else eval32<kPathMid2, FULL, PPT>(q, dtab, part, jr0);
+    } else if (misses(q.r3lo, q.r3hi, wlo, whi)) {
+        eval32<kPathMid2, FULL, PPT>(q, dtab, part, jr0);
+    } else {
+        eval32<kPathAny, FULL, PPT>(q, dtab, part, jr0);
+    }
+}
+
 template <int PPT>
-__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec) {
+__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec, float *dtab) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wlo = wave * 64 * PPT, whi = wlo + 64 * PPT;
+    const int jr0 = wlo + lane;
     const TileInfo ti = a.tiles[tile];
     const int64_t j0 = tile * W;
     float hi[PPT], lo[PPT], part[PPT];
@@ -210,20 +350,19 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec32 r;
-        r.lo = 0; r.hi = 0; r.d0 = 0; r.pad = 0;
-        r.r = 0.0f; r.vbh = 0.0f; r.vbl = 0.0f; r.y = 0.0f; r.y2 = 0.0f;
-        r.A = 0.0f; r.B = 0.0f; r.G = 0.0f;
+        r.lo = 0; r.hi = 0;
+        bool core = false;
         if (i < ti.hi) {
             const double nu = a.nu[i];
-            int64_t wlo, whi;
-            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wlo, whi);
-            const int64_t clo = max(wlo, j0), chi = min(whi, j0 + W);
+            int64_t wl, wh;
+            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
+            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
             if (clo < chi) {
                 const double ia = a.ia[i], y = a.y[i];
                 // nearest grid point to the centre (may lie off the grid)
                 const double jcd = rint((nu - a.numin) / a.dnu);
                 const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
-                const double vb = a.dnu * ia;
+                const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
                 const float vbh = (float)vb;
                 const double A = ldexp(a.S[i], -ti.e) * kISqrtPi;
                 const float yf = (float)y;
@@ -235,54 +374,55 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.A = (float)A;
                 r.B = (float)(A * y * kISqrtPi);
                 r.G = (float)(A * y * (2.0 / kPi));
-                r.d0 = (int32_t)((double)j0 - jcd);
+                r.cy = -2.0f * yf * yf;
+                r.d0 = (int32_t)d0;
                 r.lo = (int32_t)(clo - j0);
                 r.hi = (int32_t)(chi - j0);
+                r.dop = yf < kY0 ? 1 : 0;
+                const double y2 = (double)yf * (double)yf;
+                region_interval((double)kRF32, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
+                region_interval((double)kRM32, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
+                region_interval((double)kRB32, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
+                core = r.r3lo < r.r3hi && r.r3hi > 0 && r.r3lo < W;
+                if (core) core_consts32(yf, r.ey, r.q1);
+                else { r.ey = 0.0f; r.q1 = 0.0f; }
+            }
+        }
+        // the previous stage's readers of rec[] and dtab[] are done after this
+        __syncthreads();
+        rec[t] = r;
+        if (core) {
+            const double y2 = (double)r.y2;
+            for (int n = 1; n <= kNT32; ++n) {
+                const double an = kA32 * n;
+                dtab[t * kDT + n] = (float)(1.0 / fma(an, an, y2));
             }
         }
         __syncthreads();
-        rec[t] = r;
-        __syncthreads();
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
-            const Rec32 q = rec[l];
-            if (q.lo < q.hi) {
-                const bool full = q.lo == 0 && q.hi == W;
-                const float m0 = (float)(t + q.d0);
+            const Rec32 &q = rec[l];
+            const int32_t qlo = q.lo, qhi = q.hi;
+            if (qlo >= qhi) continue;
+            if (!(qhi <= wlo || qlo >= whi)) {
+                const float *dt = dtab + l * kDT;
+                if (qlo <= wlo && qhi >= whi) dispatch32<true, PPT>(q, dt, part, jr0, wlo, whi);
+                else dispatch32<false, PPT>(q, dt, part, jr0, wlo, whi);
+            }
+            if (++since == 32) {
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    const float m = m0 + (float)(kThreads * k);
-                    const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
-                    const float x2 = x * x, r2 = x2 + q.y2;
-                    const float inv = rcp32(r2);
-                    float v = q.B * inv * fmaf(fmaf(3.0f, x2, -q.y2) * 0.5f, inv * inv, 1.0f);
-                    if (__any(r2 < kRF32)) {
-                        if (r2 < kRF32) {
-                            if (r2 >= kRB32) v = q.G * gh4_sum32(x2, r2);
-                            else v = q.A * rew916_32(x, q.y);
-                        }
-                    }
-                    if (!full) {
-                        const int jr = t + kThreads * k;
-                        v = (jr >= q.lo && jr < q.hi) ? v : 0.0f;
-                    }
-                    part[k] += v;
-                }
-                if (++since == 32) {
-#pragma unroll
-                    for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
-                    since = 0;
-                }
+                for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
+                since = 0;
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         two_sum_acc(hi[k], lo[k], part[k]);
-        const int64_t j = j0 + t + kThreads * k;
+        const int64_t j = j0 + jr0 + 64 * k;
         if (j < a.nx) {
-            const double s = ldexp((double)hi[k] + (double)lo[k], ti.e);
-            a.K[j] = a.accumulate ? a.K[j] + s : a.kmin + s;
+            const double sum = ldexp((double)hi[k] + (double)lo[k], ti.e);
+            a.K[j] = a.accumulate ? a.K[j] + sum : a.kmin + sum;
         }
     }
 }
@@ -294,7 +434,8 @@ __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
         tile_f64<PPT>(a, blockIdx.x, rec);
     } else {
         __shared__ Rec32 rec[kStage];
-        tile_f32c<PPT>(a, blockIdx.x, rec);
+        __shared__ float dtab[kStage * kDT];
+        tile_f32c<PPT>(a, blockIdx.x, rec, dtab);
     }
 }
 
@@ -305,6 +446,7 @@ __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__re
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
     __shared__ int64_t s_item;
+    __shared__ float dtab[MODE == HK_VOIGT_FP64 ? 1 : kStage * kDT];
     __shared__ __attribute__((aligned(16))) char smem[(MODE == HK_VOIGT_FP64 ? sizeof(Rec64) : sizeof(Rec32)) * kStage];
     const int64_t total = item_start[nspec];
     for (;;) {
@@ -318,7 +460,7 @@ __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__re
         const VArgs a = specs[s];
         const int64_t tile = item - item_start[s];
         if (MODE == HK_VOIGT_FP64) tile_f64<PPT>(a, tile, (Rec64 *)smem);
-        else tile_f32c<PPT>(a, tile, (Rec32 *)smem);
+        else tile_f32c<PPT>(a, tile, (Rec32 *)smem, dtab);
     }
 }
 

@@ -69,6 +69,8 @@ VOIGT_CASES = [
     ("doppler-cut", 3000.0, 0.01, 2, 40.0, 300, (1000.0, 1010.0, 0.001), (998.0, 1012.0)),
     ("no-cut", 700.0, 0.5, 0, 0.0, 60, (1000.0, 1004.0, 0.002), (990.0, 1014.0)),
     ("low-T-low-P", 100.0, 1e-6, 0, 1.0, 400, (1000.0, 1002.0, 0.0001), (999.5, 1002.5)),
+    # many line cores overlap every point and every 256-line LDS stage
+    ("dense-cores", 2000.0, 9.87, 0, 5.0, 20000, (10000.0, 10010.0, 0.001), (9995.0, 10015.0)),
 ]
 
 
@@ -119,9 +121,11 @@ def test_voigt_tiles_and_determinism(E):
     sp = species_from(d)
     st = E.line_params(sp, 900.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=5.0)
     ref = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, 1024).cpu().numpy()
-    # FP64: every point adds its lines in index order whatever the tile
+    np.testing.assert_array_equal(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, 1024).cpu().numpy(), ref)
+    # other tiles: the same lines in the same order, but a wave may pick a
+    # different (equally valid) region formula for a point
     for tp in (512, 2048, 4096):
-        np.testing.assert_array_equal(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, tp).cpu().numpy(), ref)
+        assert rel_err(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64, tp).cpu().numpy(), ref) <= 1e-13
     r32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, 2048).cpu().numpy()
     np.testing.assert_array_equal(E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, 2048).cpu().numpy(), r32)
     for tp in (512, 1024, 4096):
@@ -163,5 +167,29 @@ def test_config0_fp64_and_fp32c_subsample(E):
     Ko = O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj)
     e64, e32, e3264 = rel_err(K64[jj], Ko), rel_err(K32[jj], Ko), rel_err(K32, K64)
     print("c0: FP64 vs oracle %.3e, FP32C vs oracle %.3e, FP32C vs FP64 (full grid) %.3e"
+          % (e64, e32, e3264))
+    assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
+
+
+def test_config1_precision_at_scale(E):
+    """Workload c1 (1e7 lines, 3e7 points, 2000 K, 10 bar, cut 25): FP32C vs
+    FP64 on the whole grid and both vs the oracle on 2000 random points."""
+    from heliosk import configs
+    w = configs.WORKLOADS["c1"]
+    arr = configs.species_arrays(w, 0)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    T, Pbar = w.tp[0]
+    st = E.line_params(sp, T, Pbar * configs.POBAR, Q=[1.0], cut_mode=0, cut=w.cut)
+    grid = _grid(E, w.numin, w.numax, w.dnu)
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64)
+    K32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C)
+    import torch
+    tau = 1e-12 * float(K64.max())
+    e3264 = float(torch.max(torch.abs(K32 - K64) / torch.clamp(torch.abs(K64), min=tau)))
+    jj = np.sort(np.random.default_rng(1).choice(grid.nx, 2000, replace=False))
+    Ko = O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj)
+    k64 = K64.cpu().numpy()[jj]; k32 = K32.cpu().numpy()[jj]
+    e64, e32 = rel_err(k64, Ko), rel_err(k32, Ko)
+    print("c1: FP64 vs oracle %.3e, FP32C vs oracle %.3e, FP32C vs FP64 (full grid) %.3e"
           % (e64, e32, e3264))
     assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
