@@ -73,8 +73,9 @@ def main():
         src = torch.ones(n, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
+        total = float(src.sum())          # a reduction that reads the buffer once
         torch.cuda.synchronize()
-        log("bench: copy probe of %d MiB done" % args.copy_probe_mb)
+        log("bench: copy probe of %d MiB done (%g)" % (args.copy_probe_mb, total))
         del src, dst
     t0 = time.time()
     arr = configs.species_arrays(w, 0)

@@ -164,7 +164,7 @@ __device__ __forceinline__ double core_q1(double y, double a, int N) {
     return s;
 }
 
-__device__ __noinline__ double rew_core64(double x, double y, double ey, double q1) {
+__device__ __forceinline__ double rew_core64(double x, double y, double ey, double q1) {
     constexpr double a = kA64;
     x = fabs(x);
     if (x == 0.0) return ey;

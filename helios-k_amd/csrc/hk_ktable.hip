@@ -12,7 +12,8 @@
 // Here:
 //   * bins of <= 16384 points are sorted by one workgroup each with a
 //     bitonic network in LDS; longer bins are cut into LDS-sorted chunks and
-//     merged pass by pass (merge path, each thread emitting 8 outputs);
+//     merged pass by pass (one merge-path split per workgroup, slices staged
+//     through LDS, coalesced reads and writes);
 //   * one workgroup per (spectrum, bin) applies the findCut rules, the
 //     rescale, ln, and forms the nC Chebyshev moments m_i = sum_k T_i(x_k) b_k
 //     plus the first nC values b0 of b = ln K2;
@@ -113,6 +114,168 @@ __global__ __launch_bounds__(256) void merge_pass_kernel(const double *__restric
         const bool takeA = j >= n1 || (i < n0 && A[i] <= B[j]);
         dst[k] = takeA ? A[i] : B[j];
         if (takeA) ++i; else ++j;
+    }
+}
+
+// Merge pass with LDS staging: each workgroup emits kMB consecutive outputs
+// of one merged pair; the two input slices it needs are found by one merge
+// path search per workgroup, loaded coalesced into LDS, merged there (each
+// thread kMergeE outputs) and written back coalesced.
+constexpr int kMB = 256 * kMergeE;
+
+__device__ __forceinline__ int64_t merge_split(const double *A, int64_t n0, const double *B,
+                                               int64_t n1, int64_t diag) {
+    int64_t lo = max((int64_t)0, diag - n1), hi = min(diag, n0);
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[diag - mid - 1]) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void merge_lds_kernel(const double *__restrict__ in,
+                                                        double *__restrict__ out,
+                                                        const Seg *__restrict__ segs, int64_t w) {
+    __shared__ double sA[kMB], sB[kMB], sO[kMB];
+    __shared__ int64_t sp[4];
+    const Seg sg = segs[blockIdx.y];
+    const int64_t o0 = (int64_t)blockIdx.x * kMB;
+    if (o0 >= sg.len) return;
+    const int t = threadIdx.x;
+    const int64_t p = o0 / (2 * w), ps = p * 2 * w;
+    const int64_t n0 = min(w, sg.len - ps), n1 = min(w, sg.len - ps - n0);
+    const double *A = in + sg.start + ps;
+    const double *B = A + n0;
+    const int64_t d0 = o0 - ps, d1 = min(d0 + kMB, n0 + n1);
+    if (t == 0) sp[0] = merge_split(A, n0, B, n1, d0);
+    if (t == 64) sp[1] = merge_split(A, n0, B, n1, d1);
+    __syncthreads();
+    const int64_t a0 = sp[0], a1 = sp[1], b0 = d0 - a0, b1 = d1 - a1;
+    const int na = (int)(a1 - a0), nb = (int)(b1 - b0), tot = (int)(d1 - d0);
+    for (int i = t; i < na; i += 256) sA[i] = A[a0 + i];
+    for (int i = t; i < nb; i += 256) sB[i] = B[b0 + i];
+    __syncthreads();
+    const int d = t * kMergeE;
+    if (d < tot) {
+        int lo = max(0, d - nb), hi = min(d, na);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sA[mid] <= sB[d - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        int i = lo, j = d - lo;
+        const int cnt = min(kMergeE, tot - d);
+        for (int q = 0; q < cnt; ++q) {
+            const bool takeA = j >= nb || (i < na && sA[i] <= sB[j]);
+            sO[d + q] = takeA ? sA[i] : sB[j];
+            if (takeA) ++i; else ++j;
+        }
+    }
+    __syncthreads();
+    double *dst = out + sg.start + o0;
+    for (int i = t; i < tot; i += 256) dst[i] = sO[i];
+}
+
+// ---------------------------------------------------------------------------
+// Fit, fast form for bins that are sorted in place (bin segments == the
+// fixed Nxb chunks).  findCut (resample.h:370-386) on a sorted bin reduces
+// to c = #{K <= kmin}: Nxmin = c, except in the last bin where
+// K[Nx-3] <= kmin (c >= Nxb-2) gives Nxb.  Each workgroup takes kFitChunk
+// points of one bin and writes their nC partial moments; kfit_combine sums
+// the partials in chunk order and applies P.
+// ---------------------------------------------------------------------------
+constexpr int kFitChunk = 4096;
+
+__device__ __forceinline__ int nxmin_sorted(const double *Kb, int Nxb, bool last, double kmin) {
+    int lo = 0, hi = Nxb;                  // first index with K > kmin
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (Kb[mid] <= kmin) lo = mid + 1; else hi = mid;
+    }
+    return (last && lo >= Nxb - 2) ? Nxb : lo;
+}
+
+__global__ __launch_bounds__(256) void kfit_partial_kernel(const double *__restrict__ Ksorted,
+                                                           int64_t nx, int Nxb, int nbins, int nC,
+                                                           double kmin, double unit_scale,
+                                                           double *__restrict__ part,
+                                                           double *__restrict__ b0out,
+                                                           int32_t *__restrict__ nxmin_out) {
+    __shared__ double red[kMaxNC][4];
+    __shared__ int s_nxm;
+    const int ch = blockIdx.x, b = blockIdx.y, sp = blockIdx.z, t = threadIdx.x;
+    const int nch = gridDim.x;
+    const double *Kb = Ksorted + (int64_t)sp * nx + (int64_t)b * Nxb;
+    if (t == 0) s_nxm = nxmin_sorted(Kb, Nxb, b == nbins - 1, kmin);
+    __syncthreads();
+    const int nxm = s_nxm;
+    const int64_t bid = (int64_t)sp * nbins + b;
+    if (ch == 0 && t == 0) nxmin_out[bid] = nxm;
+    double m[kMaxNC];
+#pragma unroll
+    for (int i = 0; i < kMaxNC; ++i) m[i] = 0.0;
+    const double sres = 1.0 - nxm / ((double)(Nxb - 1));
+    const double nlm1 = (double)Nxb - 1.0;
+    const int k0 = ch * kFitChunk, k1 = min(Nxb, k0 + kFitChunk);
+    for (int k = k0 + t; k < k1; k += 256) {
+        double Kv;
+        if (nxm > 0) {
+            double ii = fma(sres, (double)k, (double)nxm);
+            if (ii >= Nxb - 1) ii = 0.999999 * (Nxb - 1);
+            const int il = (int)(ii / ((double)Nxb) * Nxb);
+            const double Kl = Kb[il], Kr = Kb[il + 1];
+            Kv = fma(Kr - Kl, ii - il, Kl);
+        } else {
+            Kv = Kb[k];
+        }
+        const double bv = log(Kv * unit_scale);
+        if (k < nC) b0out[bid * nC + k] = bv;
+        const double x = k * 2.0 / nlm1 - 1.0;
+        double T0 = 1.0, T1 = x;
+        m[0] += bv;
+        if (nC > 1) m[1] = fma(x, bv, m[1]);
+#pragma unroll
+        for (int i = 2; i < kMaxNC; ++i) {
+            if (i < nC) {
+                const double T2 = fma(2.0 * x, T1, -T0);
+                m[i] = fma(T2, bv, m[i]);
+                T0 = T1; T1 = T2;
+            }
+        }
+    }
+    // wave sums (fixed butterfly), then the 4 waves in order
+    const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int i = 0; i < kMaxNC; ++i) {
+        if (i < nC) {
+            double v = m[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[i][wv] = v;
+        }
+    }
+    __syncthreads();
+    if (t < nC) part[(bid * nch + ch) * nC + t] = ((red[t][0] + red[t][1]) + red[t][2]) + red[t][3];
+}
+
+__global__ __launch_bounds__(64) void kfit_combine_kernel(const double *__restrict__ part,
+                                                          const double *__restrict__ b0in, int nch,
+                                                          int nC, const double *__restrict__ P,
+                                                          double *__restrict__ coeff) {
+    __shared__ double mom[kMaxNC];
+    const int64_t bid = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const int t = threadIdx.x;
+    if (t < nC) {
+        double s = 0.0;
+        for (int c = 0; c < nch; ++c) s += part[(bid * nch + c) * nC + t];
+        mom[t] = s;
+    }
+    __syncthreads();
+    if (t < nC) {
+        const double *Pr = P + (int64_t)t * 2 * nC;
+        const double *b0 = b0in + bid * nC;
+        double c = 0.0;
+        for (int i = 0; i < nC; ++i) c = fma(Pr[i], mom[i], c);
+        for (int i = 0; i < nC; ++i) c = fma(Pr[nC + i], b0[i], c);
+        coeff[bid * nC + t] = c;
     }
 }
 
@@ -429,8 +592,9 @@ static int get_plan(int Nxb, int nC, std::vector<double> &P) {
 
 // Workspace layout.
 struct KtLayout {
-    size_t off_seg, off_plan, off_hv, off_segs, off_sorted, off_tmp, total;
+    size_t off_seg, off_plan, off_hv, off_segs, off_part, off_b0, off_sorted, off_tmp, total;
     bool householder;
+    int nch;
 };
 
 static bool use_householder(int64_t Nxb, int nC) { return Nxb < (int64_t)kPlanMinRatio * nC; }
@@ -445,6 +609,9 @@ static KtLayout kt_layout(int64_t nx, int nbins, int nC, int nspec) {
     L.off_hv = o;
     if (L.householder) o = align_up(o + sizeof(double) * (size_t)(Nxb * nC + 2 * nC), 256);
     L.off_segs = o; o = align_up(o + sizeof(Seg) * (size_t)nspec * (nbins + 1), 256);
+    L.nch = (int)((Nxb + kFitChunk - 1) / kFitChunk);
+    L.off_part = o; o = align_up(o + sizeof(double) * (size_t)nspec * nbins * L.nch * nC, 256);
+    L.off_b0 = o; o = align_up(o + sizeof(double) * (size_t)nspec * nbins * nC, 256);
     L.off_sorted = o; o = align_up(o + sizeof(double) * (size_t)nspec * nx, 256);
     L.off_tmp = o; o = align_up(o + sizeof(double) * (size_t)nspec * nx, 256);
     L.total = o + 256;
@@ -571,6 +738,11 @@ extern "C" int hk_ktable(const double *K, int nspec, const hk_grid *g, double nu
     if (int e = hk_bin_segments(g, numax, nbins, hseg.data())) return e;
     int64_t maxlen = 0;
     for (int b = 0; b <= nbins; ++b) maxlen = std::max(maxlen, hseg[b + 1] - hseg[b]);
+    // segments equal to the fixed Nxb chunks (the usual case): sorted bins can
+    // use the chunked fit; otherwise the literal per-bin rules
+    bool aligned = true;
+    for (int b = 0; b <= nbins; ++b)
+        if (hseg[b] != (int64_t)b * Nxb) aligned = false;
     if (maxlen <= kSortLDS) {
         hipLaunchKernelGGL((bitonic_kernel<kSortLDS>), dim3(1, nsegs), dim3(kSortThreads), 0, s, K,
                            sorted, segs, (int64_t)kSortLDS);
@@ -588,17 +760,27 @@ extern "C" int hk_ktable(const double *K, int nspec, const hk_grid *g, double nu
         HK_LAUNCH_CHECK();
         double *src = first, *dst = other;
         for (int64_t w = chunk; w < maxlen; w *= 2) {
-            const int64_t per_block = 256 * kMergeE;
-            hipLaunchKernelGGL(merge_pass_kernel, dim3((unsigned)((maxlen + per_block - 1) / per_block), nsegs),
+            hipLaunchKernelGGL(merge_lds_kernel, dim3((unsigned)((maxlen + kMB - 1) / kMB), nsegs),
                                dim3(256), 0, s, src, dst, segs, w);
             HK_LAUNCH_CHECK();
             std::swap(src, dst);
         }
     }
     const double *HV = L.householder ? (const double *)(base + L.off_hv) : nullptr;
-    hipLaunchKernelGGL(kfit_kernel, dim3(nbins, nspec), dim3(kFitThreads), 0, s, sorted, g->nx, Nxb,
-                       nbins, nC, kmin, unit_scale, P, HV, coeff, nxmin);
-    HK_LAUNCH_CHECK();
+    if (aligned && !L.householder && Nxb >= 4) {
+        double *part = (double *)(base + L.off_part);
+        double *b0 = (double *)(base + L.off_b0);
+        hipLaunchKernelGGL(kfit_partial_kernel, dim3(L.nch, nbins, nspec), dim3(256), 0, s, sorted,
+                           g->nx, Nxb, nbins, nC, kmin, unit_scale, part, b0, nxmin);
+        HK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(kfit_combine_kernel, dim3(nbins, nspec), dim3(64), 0, s, part, b0, L.nch,
+                           nC, P, coeff);
+        HK_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(kfit_kernel, dim3(nbins, nspec), dim3(kFitThreads), 0, s, sorted, g->nx,
+                           Nxb, nbins, nC, kmin, unit_scale, P, HV, coeff, nxmin);
+        HK_LAUNCH_CHECK();
+    }
     return HK_OK;
 }
 

@@ -7,7 +7,8 @@
 //   S = S_file/Q*Ab*Sscale * exp(-c2 E/T + c2 E/T0)
 //       * (1 - exp(-c2 nu/T)) / (1 - exp(-c2 nu/T0)) * (1/alpha_D)
 //   alpha_L = vy (P/1 atm) (T0/T)^n + A/(4 pi c),  y = alpha_L / alpha_D
-// and the cut half width of S3_kernel (voigt.h:442-450) in cm^-1.  Each
+// and the cut half width of S3_kernel (voigt.h:442-450) in cm^-1, plus
+// erfcx(y) for the line-core series of the Voigt kernels.  Each
 // workgroup also writes the block maxima the Voigt kernels use for their
 // line-range search radius and FP32 scaling.
 #include <math.h>
@@ -28,13 +29,14 @@ __global__ __launch_bounds__(256) void line_params_kernel(
     const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTab tab,
     int64_t n, double T, double P, int cut_mode, double cut,
     double *__restrict__ o_nu, double *__restrict__ o_S, double *__restrict__ o_ia,
-    double *__restrict__ o_y, double *__restrict__ o_hw, double *__restrict__ blk_max,
-    double *__restrict__ blk_rad) {
+    double *__restrict__ o_y, double *__restrict__ o_hw, double *__restrict__ o_ey,
+    double *__restrict__ blk_max,
+    double *__restrict__ blk_rad, double *__restrict__ blk_core) {
     using namespace hk;
-    __shared__ double smax[256], srad[256];
+    __shared__ double smax[256], srad[256], score[256];
     const int t = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * 256 + t;
-    double smx = 0.0, rad = 0.0;
+    double smx = 0.0, rad = 0.0, crad = 0.0;
     if (i < n) {
         const double c = kH * kC / kKB;
         int k = iso ? min(max(iso[i], 0), HK_MAX_ISO - 1) : 0;
@@ -51,22 +53,31 @@ __global__ __launch_bounds__(256) void line_params_kernel(
         double hw = cut_mode == 1 ? cut * aL : (cut_mode == 2 ? cut / ia : cut);
         if (nu == 0.0) { S = 0.0; ia = 0.0; y = 0.0; hw = -1.0; }
         o_nu[i] = nu; o_S[i] = S; o_ia[i] = ia; o_y[i] = y; o_hw[i] = hw;
+        o_ey[i] = erfcx(y);
         smx = fabs(S);
         rad = hw > 0.0 ? hw + fabs(nu - key) : 0.0;
+        // the line core x^2 + y^2 < 100 lies within 10 alpha_D of the centre
+        // and is empty for y >= 10 (10.5: margin for rounding in x)
+        crad = (hw > 0.0 && y < 10.5) ? fmin(hw, 10.5 / ia) + fabs(nu - key) : 0.0;
     }
-    smax[t] = smx; srad[t] = rad;
+    smax[t] = smx; srad[t] = rad; score[t] = crad;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s) {
             smax[t] = fmax(smax[t], smax[t + s]);
             srad[t] = fmax(srad[t], srad[t + s]);
+            score[t] = fmax(score[t], score[t + s]);
         }
         __syncthreads();
     }
-    if (t == 0) { blk_max[blockIdx.x] = smax[0]; blk_rad[blockIdx.x] = srad[0]; }
+    if (t == 0) {
+        blk_max[blockIdx.x] = smax[0];
+        blk_rad[blockIdx.x] = srad[0];
+        blk_core[blockIdx.x] = score[0];
+    }
 }
 
-// radius[0] = max over blocks (one workgroup, fixed order: deterministic).
+// out[0] = max over blocks (one workgroup, fixed order: deterministic).
 __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *__restrict__ v,
                                                           int64_t n, double *__restrict__ out) {
     __shared__ double s[1024];
@@ -86,8 +97,8 @@ __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *__restri
 extern "C" size_t hk_line_state_bytes(int64_t nlines) {
     if (nlines < 0) return 0;
     int64_t nb = (nlines + hk::kBlockLines - 1) / hk::kBlockLines;
-    // nu, S, ialphaD, y, hw  +  blk_max  +  radius[0] and per-block radii
-    return (size_t)(5 * nlines + nb + 1 + nb) * sizeof(double) + 256;
+    // nu, S, ialphaD, y, hw, ey  +  blk_max  +  radius[0..1] and per-block radii
+    return (size_t)(6 * nlines + nb + 2 + 2 * nb) * sizeof(double) + 256;
 }
 
 extern "C" int hk_line_state_init(hk_line_state *st, int64_t nlines, void *mem, size_t bytes) {
@@ -104,8 +115,11 @@ extern "C" int hk_line_state_init(hk_line_state *st, int64_t nlines, void *mem, 
     st->ialphaD = p; p += nlines;
     st->y = p; p += nlines;
     st->hw = p; p += nlines;
+    st->ey = p; p += nlines;
     st->blk_max = p; p += nb;
-    st->radius = p;          // radius[0], then nb per-block radii
+    // radius[0]: cut reach, radius[1]: line-core reach; then the nb per-block
+    // cut reaches and the nb per-block core reaches
+    st->radius = p;
     st->key = nullptr;
     return HK_OK;
 }
@@ -126,9 +140,11 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
     double cutv = cut->cut == 0.0 ? 1.0e30 : cut->cut;    // heliosk.cu:941
     hipStream_t s = (hipStream_t)stream;
     if (n == 0) {
-        hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 1,
-                           (int64_t)0, st->radius);
-        HK_LAUNCH_CHECK();
+        for (int r = 0; r < 2; ++r) {
+            hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 2,
+                               (int64_t)0, st->radius + r);
+            HK_LAUNCH_CHECK();
+        }
         return HK_OK;
     }
     IsoTab tab;
@@ -141,11 +157,13 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
     }
     hipLaunchKernelGGL(line_params_kernel, dim3((unsigned)nb), dim3(256), 0, s, L->nu, L->S,
                        L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, n, tp->T, tp->P,
-                       cut->cut_mode, cutv, st->nu, st->S, st->ialphaD, st->y, st->hw,
-                       st->blk_max, st->radius + 1);
+                       cut->cut_mode, cutv, st->nu, st->S, st->ialphaD, st->y, st->hw, st->ey,
+                       st->blk_max, st->radius + 2, st->radius + 2 + nb);
     HK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 1, nb,
-                       st->radius);
-    HK_LAUNCH_CHECK();
+    for (int r = 0; r < 2; ++r) {
+        hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 2 + r * nb,
+                           nb, st->radius + r);
+        HK_LAUNCH_CHECK();
+    }
     return HK_OK;
 }

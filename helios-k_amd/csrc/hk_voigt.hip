@@ -13,6 +13,17 @@
 //   * a line contributes to point j iff |x_j - nu| < hw in FP64 (the
 //     integer window is computed once per line and tile, cut_window).
 //
+// Two passes per spectrum, in stream order:
+//   * the main pass above evaluates every (line, point) pair of the cut
+//     windows except the line cores, x^2 + y^2 < R_core, which it skips;
+//   * the line-core pass (voigt_core_kernel) adds the core pairs: tiles of
+//     256 points, one wave per 64 points, over only the lines whose core can
+//     reach the tile (a second binary search with the core reach), compacted
+//     in LDS in line order.  Both passes form x and r2 with the same
+//     operations, so each pair is evaluated by exactly one of them.
+//   Keeping the core series out of the main kernel keeps its register
+//   footprint, and so its occupancy, set by the wing formulas.
+//
 // Forms:
 //   HK_VOIGT_FP64  x = (x_j - nu)/alpha_D in FP64, FP64 profile (Algorithm
 //                  916 core, 8- and 4-node Gauss-Hermite, 3-term asymptotic)
@@ -46,8 +57,10 @@ struct TileInfo {
 };
 
 struct VArgs {
-    const double *nu, *S, *ia, *y, *hw;
-    const TileInfo *tiles;
+    const double *nu, *S, *ia, *y, *hw, *ey;
+    const double *key, *blk_max, *radius;   // line-range search (batched form)
+    int64_t nl;
+    TileInfo *tiles;
     double numin, dnu;
     int64_t nx;
     double kmin;
@@ -61,13 +74,25 @@ struct VArgs {
 // core).  A wave whose points miss an interval takes the branch-free path of
 // the next region out.
 struct Rec64 {
-    double nu, ia, y, y2, A, B, G, p2, p3, p4, ey, q1;
+    double nu, ia, y2, B, G, p2, p3, p4;
     int32_t lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi;
 };
 
 struct Rec32 {
-    float r, vbh, vbl, y, y2, A, B, G, cy, ey, q1, pad2;
+    float r, vbh, vbl, y2, A, B, G, cy;
     int32_t d0, lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi, dop, pad0, pad1;
+};
+
+// Line-core pass records: lo/hi bound the points of the tile that lie both
+// in the line's cut window and in its (conservative) core interval.
+struct Core64 {
+    double nu, ia, y, y2, A, ey, q1;
+    int32_t lo, hi;
+};
+
+struct Core32 {
+    float r, vbh, vbl, y, y2, A, ey, q1;
+    int32_t d0, lo, hi, pad;
 };
 
 __device__ __forceinline__ int64_t lower_bound_dev(const double *v, int64_t n, double key) {
@@ -88,14 +113,13 @@ __device__ __forceinline__ int64_t upper_bound_dev(const double *v, int64_t n, d
     return lo;
 }
 
-// One wavefront per tile.
-__global__ __launch_bounds__(256) void tile_ranges_kernel(
-    const double *__restrict__ key, int64_t nl, const double *__restrict__ radius,
-    const double *__restrict__ blk_max, double numin, double dnu, int64_t nx, int tile_pts,
-    int64_t ntiles, int scale, TileInfo *__restrict__ tiles) {
+// Line range (and FP32 scale) of one tile, computed by one wavefront.
+__device__ __forceinline__ void tile_range(const double *__restrict__ key, int64_t nl,
+                                           const double *__restrict__ radius,
+                                           const double *__restrict__ blk_max, double numin,
+                                           double dnu, int64_t nx, int tile_pts, int64_t tile,
+                                           int scale, TileInfo *__restrict__ tiles) {
     const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;
     const double R = radius[0];
     const int64_t j0 = tile * tile_pts;
     const int64_t j1 = min(j0 + (int64_t)tile_pts, nx) - 1;
@@ -121,6 +145,38 @@ __global__ __launch_bounds__(256) void tile_ranges_kernel(
     }
 }
 
+__global__ __launch_bounds__(256) void tile_ranges_kernel(
+    const double *__restrict__ key, int64_t nl, const double *__restrict__ radius,
+    const double *__restrict__ blk_max, double numin, double dnu, int64_t nx, int tile_pts,
+    int64_t ntiles, int scale, TileInfo *__restrict__ tiles) {
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    tile_range(key, nl, radius, blk_max, numin, dnu, nx, tile_pts, tile, scale, tiles);
+}
+
+// Spectrum of a queue item: item_start[s] <= item < item_start[s + 1].
+__device__ __forceinline__ int item_spectrum(const int64_t *__restrict__ item_start, int nspec,
+                                             int64_t item) {
+    int lo = 0, hi = nspec - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (item_start[m] <= item) lo = m; else hi = m - 1;
+    }
+    return lo;
+}
+
+// Batched form: tile ranges of every (spectrum, tile) item, one wave each.
+__global__ __launch_bounds__(256) void batched_ranges_kernel(const VArgs *__restrict__ specs,
+                                                             const int64_t *__restrict__ item_start,
+                                                             int nspec, int tile_pts, int scale) {
+    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= item_start[nspec]) return;
+    const int s = item_spectrum(item_start, nspec, item);
+    const VArgs &a = specs[s];
+    tile_range(a.key, a.nl, a.radius, a.blk_max, a.numin, a.dnu, a.nx, tile_pts,
+               item - item_start[s], scale, a.tiles);
+}
+
 // Conservative tile-relative interval of the points with x^2 + y^2 < R, for
 // x_j ~ (j - jc) vb + rr and tile start j0 = jc + d0 ... as index j - j0.
 __device__ __forceinline__ void region_interval(double R, double y2, double rr, double vb,
@@ -137,13 +193,39 @@ __device__ __forceinline__ bool misses(int32_t lo, int32_t hi, int wlo, int whi)
     return hi <= wlo || lo >= whi;
 }
 
-__device__ __noinline__ void core_consts64(double y, double &ey, double &q1) {
-    ey = erfcx(y);
-    q1 = core_q1(y, kA64, kN64);
+// FP64 per-line core constant q1 = sum_{n=1..13} e^{-a^2n^2} / (a^2 n^2 +
+// y^2), a = kA64, with e^{-a^2 n^2} and a^2 n^2 as literals.
+__device__ __forceinline__ double core_q1_64(double y) {
+    constexpr double c[13] = {0.755683658862919, 0.32610700211508303, 0.08036376573837029,
+                              0.011309424249254312, 0.0009088679210029174, 4.171008895272509e-05,
+                              1.0931040894428156e-06, 1.6359197067600175e-08, 1.3981137635663895e-10,
+                              6.823435648847615e-13, 1.9017076395301824e-15, 3.0266657632272252e-18,
+                              2.7508394197950315e-21};
+    constexpr double an2[13] = {0.2801324310308401, 1.1205297241233605, 2.5211918792775614,
+                                4.482118896493442, 7.003310775771004, 10.084767517110246,
+                                13.726489120511166, 17.928475585973768, 22.690726913498057,
+                                28.013243103084015, 33.89602415473165, 40.33907006844098,
+                                47.34238084421199};
+    const double yy = y * y;
+    double s = 0.0;
+#pragma unroll
+    for (int n = 0; n < 13; ++n) s = fma(c[n], rcp64(an2[n] + yy), s);
+    return s;
 }
-__device__ __noinline__ void core_consts32(float y, float &ey, float &q1) {
-    ey = (float)erfcx((double)y);
-    q1 = (float)core_q1((double)y, kA32, kN32);
+
+// FP32 per-line core constant q1 = sum_{n=1..6} e^{-a^2n^2} / (a^2 n^2 + y^2),
+// a = kA32, with e^{-a^2 n^2} and a^2 n^2 as literals.
+__device__ __forceinline__ float core_q1_32(float y) {
+    constexpr float c[6] = {0.5559463030549833f, 0.09552815420260209f, 0.005073359286059877f,
+                            8.327709087244148e-05f, 4.2249460229375276e-07f,
+                            6.624948055035271e-10f};
+    constexpr float an2[6] = {0.5870835666346752f, 2.348334266538701f, 5.283752099712076f,
+                              9.393337066154803f, 14.67708916586688f, 21.135008398848303f};
+    const float yy = y * y;
+    float s = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) s = fmaf(c[n], rcp32(an2[n] + yy), s);
+    return s;
 }
 
 // Paths of the per-(line, wave) dispatch.
@@ -180,7 +262,7 @@ __device__ __forceinline__ void eval64(const Rec64 &qr, const double *xj, double
             } else if (r2 >= kRB64) {
                 v = G * gh8_sum64(x * x, r2);
             } else {
-                v = qr.A * rew_core64(x, qr.y, qr.ey, qr.q1);
+                v = 0.0;            // line core: added by the line-core pass
             }
         }
         if (!FULL) {
@@ -228,8 +310,8 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
             if (clo < chi) {
                 const double ia = a.ia[i], y = a.y[i], y2 = y * y;
                 const double A = a.S[i] * kISqrtPi;
-                r.nu = nu; r.ia = ia; r.y = y; r.y2 = y2;
-                r.A = A; r.B = A * y * kISqrtPi; r.G = A * y * (2.0 / kPi);
+                r.nu = nu; r.ia = ia; r.y2 = y2;
+                r.B = A * y * kISqrtPi; r.G = A * y * (2.0 / kPi);
                 r.p2 = 3.75 - 2.0 * y2; r.p3 = -15.0 * y2; r.p4 = 12.0 * y2 * y2;
                 r.lo = (int32_t)(clo - j0); r.hi = (int32_t)(chi - j0);
                 const double jcd = rint((nu - a.numin) / a.dnu);
@@ -238,11 +320,6 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
                 region_interval(kRF64, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
                 region_interval(kRM64, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
                 region_interval(kRB64, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
-                if (r.r3lo < r.r3hi && r.r3hi > 0 && r.r3lo < W) {
-                    core_consts64(y, r.ey, r.q1);
-                } else {
-                    r.ey = 0.0; r.q1 = 0.0;
-                }
             }
         }
         __syncthreads();
@@ -278,7 +355,7 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
 constexpr int kDT = 16;      // per-line core table stride (entries 1..kNT32 used)
 
 template <int PATH, bool FULL, int PPT>
-__device__ __forceinline__ void eval32(const Rec32 &qr, const float *dtab, float *part, int jr0) {
+__device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
     const float r0 = qr.r, vbh = qr.vbh, vbl = qr.vbl, y2 = qr.y2, B = qr.B, G = qr.G, cy = qr.cy;
     const int32_t lo = qr.lo, hi = qr.hi;
     const float m0 = (float)(jr0 + qr.d0);
@@ -303,7 +380,7 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, const float *dtab, float
                 v = G * gh8_sum32(x * x, r2);
                 if (qr.dop) v = fmaf(qr.A, __expf(-x * x), v);
             } else {
-                v = qr.A * rew_core32(x, qr.y, qr.ey, qr.q1, dtab);
+                v = 0.0f;           // line core: added by the line-core pass
             }
         }
         if (!FULL) {
@@ -315,26 +392,25 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, const float *dtab, float
 }
 
 template <bool FULL, int PPT>
-__device__ __forceinline__ void dispatch32(const Rec32 &q, const float *dtab, float *part, int jr0,
-                                           int wlo, int whi) {
+__device__ __forceinline__ void dispatch32(const Rec32 &q, float *part, int jr0, int wlo, int whi) {
     if (misses(q.r1lo, q.r1hi, wlo, whi)) {
-        eval32<kPathFar, FULL, PPT>(q, dtab, part, jr0);
+        eval32<kPathFar, FULL, PPT>(q, part, jr0);
     } else if (misses(q.r2lo, q.r2hi, wlo, whi)) {
         // 4-node rule for the whole wave unless x is so large that its
         // single-reciprocal form could overflow FP32
         const float m0 = (float)(wlo + q.d0), m1 = (float)(whi - 1 + q.d0);
         const float x0 = fmaf(m0, q.vbh, q.r), x1 = fmaf(m1, q.vbh, q.r);
-        if (fmaxf(x0 * x0, x1 * x1) < 1.0e9f) eval32<kPathMid, FULL, PPT>(q, dtab, part, jr0);
-        else eval32<kPathMid2, FULL, PPT>(q, dtab, part, jr0);
+        if (fmaxf(x0 * x0, x1 * x1) < 1.0e9f) eval32<kPathMid, FULL, PPT>(q, part, jr0);
+        else eval32<kPathMid2, FULL, PPT>(q, part, jr0);
     } else if (misses(q.r3lo, q.r3hi, wlo, whi)) {
-        eval32<kPathMid2, FULL, PPT>(q, dtab, part, jr0);
+        eval32<kPathMid2, FULL, PPT>(q, part, jr0);
     } else {
-        eval32<kPathAny, FULL, PPT>(q, dtab, part, jr0);
+        eval32<kPathAny, FULL, PPT>(q, part, jr0);
     }
 }
 
 template <int PPT>
-__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec, float *dtab) {
+__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -351,7 +427,6 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         const int64_t i = base + t;
         Rec32 r;
         r.lo = 0; r.hi = 0;
-        bool core = false;
         if (i < ti.hi) {
             const double nu = a.nu[i];
             int64_t wl, wh;
@@ -369,7 +444,6 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.r = (float)rr;
                 r.vbh = vbh;
                 r.vbl = (float)(vb - (double)vbh);
-                r.y = yf;
                 r.y2 = yf * yf;
                 r.A = (float)A;
                 r.B = (float)(A * y * kISqrtPi);
@@ -383,21 +457,11 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 region_interval((double)kRF32, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
                 region_interval((double)kRM32, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
                 region_interval((double)kRB32, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
-                core = r.r3lo < r.r3hi && r.r3hi > 0 && r.r3lo < W;
-                if (core) core_consts32(yf, r.ey, r.q1);
-                else { r.ey = 0.0f; r.q1 = 0.0f; }
             }
         }
-        // the previous stage's readers of rec[] and dtab[] are done after this
+        // the previous stage's readers of rec[] are done after this
         __syncthreads();
         rec[t] = r;
-        if (core) {
-            const double y2 = (double)r.y2;
-            for (int n = 1; n <= kNT32; ++n) {
-                const double an = kA32 * n;
-                dtab[t * kDT + n] = (float)(1.0 / fma(an, an, y2));
-            }
-        }
         __syncthreads();
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
@@ -405,9 +469,8 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             const int32_t qlo = q.lo, qhi = q.hi;
             if (qlo >= qhi) continue;
             if (!(qhi <= wlo || qlo >= whi)) {
-                const float *dt = dtab + l * kDT;
-                if (qlo <= wlo && qhi >= whi) dispatch32<true, PPT>(q, dt, part, jr0, wlo, whi);
-                else dispatch32<false, PPT>(q, dt, part, jr0, wlo, whi);
+                if (qlo <= wlo && qhi >= whi) dispatch32<true, PPT>(q, part, jr0, wlo, whi);
+                else dispatch32<false, PPT>(q, part, jr0, wlo, whi);
             }
             if (++since == 32) {
 #pragma unroll
@@ -427,6 +490,172 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Line-core pass.  Tile of kCoreW = 256 points; wave w owns the 64 points
+// [64w, 64w + 64), lane l point 64w + l.  Per stage, each thread builds the
+// record of one line of the tile's core range; the lines whose core meets
+// the tile are compacted into LDS in line order (wave ballots), then each
+// wave adds the records that overlap its points.  x and r2 are formed
+// exactly as in the main pass, so r2 < R_core selects the complementary
+// pairs.
+// ---------------------------------------------------------------------------
+constexpr int kCoreW = 256;
+
+// Position of this thread's valid record among the workgroup's valid
+// records (line order) and their total; cnt[] is 4 ints of LDS.
+__device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(valid);
+    const int below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) cnt[wave] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int c = cnt[w];
+        off += w < wave ? c : 0;
+        total += c;
+    }
+    return off + below;
+}
+
+__device__ __forceinline__ void core_f64(const VArgs &a, int64_t tile, Core64 *rec, int *cnt) {
+    constexpr int W = kCoreW;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wlo = wave * 64, whi = wlo + 64;
+    const int jr = wlo + lane;
+    const TileInfo ti = a.tiles[tile];
+    const int64_t j0 = tile * W;
+    const double xj = fma((double)(j0 + jr), a.dnu, a.numin);
+    double acc = 0.0;
+    bool touched = false;
+    for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
+        const int64_t i = base + t;
+        Core64 r;
+        bool valid = false;
+        if (i < ti.hi) {
+            const double nu = a.nu[i];
+            int64_t wl, wh;
+            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
+            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
+            const double y = a.y[i], y2 = y * y;
+            if (clo < chi && y2 < kRB64) {
+                const double ia = a.ia[i];
+                const double jcd = rint((nu - a.numin) / a.dnu);
+                const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
+                int32_t clo3, chi3;
+                region_interval(kRB64, y2, rr, a.dnu * ia, (double)j0 - jcd, W, clo3, chi3);
+                r.lo = max((int32_t)(clo - j0), max(clo3, 0));
+                r.hi = min((int32_t)(chi - j0), min(chi3, W));
+                valid = r.lo < r.hi;
+                if (valid) {
+                    r.nu = nu; r.ia = ia; r.y = y; r.y2 = y2;
+                    r.A = a.S[i] * kISqrtPi;
+                    r.ey = a.ey[i];
+                    r.q1 = core_q1_64(y);
+                }
+            }
+        }
+        // the previous stage's readers of rec[] and cnt[] are done after this
+        __syncthreads();
+        int n;
+        const int slot = compact_slot(valid, cnt, n);
+        if (valid) rec[slot] = r;
+        __syncthreads();
+        for (int l = 0; l < n; ++l) {
+            const Core64 &q = rec[l];
+            if (q.hi <= wlo || q.lo >= whi) continue;
+            touched = true;
+            const double x = (xj - q.nu) * q.ia;
+            const double r2 = fma(x, x, q.y2);
+            if (jr >= q.lo && jr < q.hi && r2 < kRB64) acc += q.A * rew_core64(x, q.y, q.ey, q.q1);
+        }
+    }
+    const int64_t j = j0 + jr;
+    if (touched && j < a.nx) a.K[j] += acc;
+}
+
+__device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *rec, float *dtab,
+                                          int *cnt) {
+    constexpr int W = kCoreW;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wlo = wave * 64, whi = wlo + 64;
+    const int jr = wlo + lane;
+    const TileInfo ti = a.tiles[tile];
+    const int64_t j0 = tile * W;
+    float hi = 0.0f, lo = 0.0f, part = 0.0f;
+    int since = 0;
+    bool touched = false;
+    for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
+        const int64_t i = base + t;
+        Core32 r;
+        bool valid = false;
+        if (i < ti.hi) {
+            const double nu = a.nu[i];
+            int64_t wl, wh;
+            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
+            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
+            const double y = a.y[i];
+            const float yf = (float)y;
+            const double y2 = (double)yf * (double)yf;
+            if (clo < chi && y2 < (double)kRB32) {
+                const double ia = a.ia[i];
+                const double jcd = rint((nu - a.numin) / a.dnu);
+                const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
+                const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
+                int32_t clo3, chi3;
+                region_interval((double)kRB32, y2, rr, vb, d0, W, clo3, chi3);
+                r.lo = max((int32_t)(clo - j0), max(clo3, 0));
+                r.hi = min((int32_t)(chi - j0), min(chi3, W));
+                valid = r.lo < r.hi;
+                if (valid) {
+                    const float vbh = (float)vb;
+                    r.r = (float)rr;
+                    r.vbh = vbh;
+                    r.vbl = (float)(vb - (double)vbh);
+                    r.y = yf;
+                    r.y2 = yf * yf;
+                    r.A = (float)(ldexp(a.S[i], -ti.e) * kISqrtPi);
+                    r.ey = (float)a.ey[i];
+                    r.q1 = core_q1_32(yf);
+                    r.d0 = (int32_t)d0;
+                }
+            }
+        }
+        // the previous stage's readers of rec[], dtab[] and cnt[] are done after this
+        __syncthreads();
+        int n;
+        const int slot = compact_slot(valid, cnt, n);
+        if (valid) {
+            rec[slot] = r;
+            for (int k = 1; k <= kNT32; ++k) {
+                const double an = kA32 * k;
+                dtab[slot * kDT + k] = (float)(1.0 / fma(an, an, (double)r.y2));
+            }
+        }
+        __syncthreads();
+        for (int l = 0; l < n; ++l) {
+            const Core32 &q = rec[l];
+            if (q.hi <= wlo || q.lo >= whi) continue;
+            touched = true;
+            const float m = (float)(jr + q.d0);
+            const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
+            const float r2 = fmaf(x, x, q.y2);
+            if (jr >= q.lo && jr < q.hi && r2 < kRB32)
+                part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
+            if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+        }
+    }
+    two_sum_acc(hi, lo, part);
+    const int64_t j = j0 + jr;
+    if (touched && j < a.nx) a.K[j] += ldexp((double)hi + (double)lo, ti.e);
+}
+
 template <int MODE, int PPT>
 __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
     if (MODE == HK_VOIGT_FP64) {
@@ -434,20 +663,37 @@ __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
         tile_f64<PPT>(a, blockIdx.x, rec);
     } else {
         __shared__ Rec32 rec[kStage];
+        tile_f32c<PPT>(a, blockIdx.x, rec);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
+    __shared__ int cnt[kThreads / 64];
+    if (MODE == HK_VOIGT_FP64) {
+        __shared__ Core64 rec[kStage];
+        core_f64(a, blockIdx.x, rec, cnt);
+    } else {
+        __shared__ Core32 rec[kStage];
         __shared__ float dtab[kStage * kDT];
-        tile_f32c<PPT>(a, blockIdx.x, rec, dtab);
+        core_f32c(a, blockIdx.x, rec, dtab, cnt);
     }
 }
 
 // Persistent form: workgroups take (spectrum, tile) items from a queue.
-// item_start[s] = first item of spectrum s (nspec+1 entries).
+// item_start[s] = first item of spectrum s (nspec+1 entries).  PPT == 0
+// selects the line-core pass.
 template <int MODE, int PPT>
 __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__restrict__ specs,
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
+    constexpr size_t kRecBytes =
+        PPT == 0 ? (MODE == HK_VOIGT_FP64 ? sizeof(Core64) : sizeof(Core32))
+                 : (MODE == HK_VOIGT_FP64 ? sizeof(Rec64) : sizeof(Rec32));
     __shared__ int64_t s_item;
-    __shared__ float dtab[MODE == HK_VOIGT_FP64 ? 1 : kStage * kDT];
-    __shared__ __attribute__((aligned(16))) char smem[(MODE == HK_VOIGT_FP64 ? sizeof(Rec64) : sizeof(Rec32)) * kStage];
+    __shared__ int cnt[kThreads / 64];
+    __shared__ float dtab[(PPT == 0 && MODE == HK_VOIGT_FP32C) ? kStage * kDT : 1];
+    __shared__ __attribute__((aligned(16))) char smem[kRecBytes * kStage];
     const int64_t total = item_start[nspec];
     for (;;) {
         __syncthreads();
@@ -455,16 +701,22 @@ __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__re
         __syncthreads();
         const int64_t item = s_item;
         if (item >= total) break;
-        int s = 0;
-        while (s + 1 < nspec && item_start[s + 1] <= item) ++s;
+        const int s = item_spectrum(item_start, nspec, item);
         const VArgs a = specs[s];
         const int64_t tile = item - item_start[s];
-        if (MODE == HK_VOIGT_FP64) tile_f64<PPT>(a, tile, (Rec64 *)smem);
-        else tile_f32c<PPT>(a, tile, (Rec32 *)smem, dtab);
+        if constexpr (PPT == 0) {
+            if constexpr (MODE == HK_VOIGT_FP64) core_f64(a, tile, (Core64 *)smem, cnt);
+            else core_f32c(a, tile, (Core32 *)smem, dtab, cnt);
+        } else {
+            if constexpr (MODE == HK_VOIGT_FP64) tile_f64<PPT>(a, tile, (Rec64 *)smem);
+            else tile_f32c<PPT>(a, tile, (Rec32 *)smem);
+        }
     }
 }
 
-__global__ void reset_counter_kernel(unsigned long long *c) { *c = 0ull; }
+__global__ void reset_counters_kernel(unsigned long long *c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+}
 
 // Algorithmic evaluation count: per line, the points of its cut window, split
 // by the reference's regions (voigt.h:1223-1230, 1490, 1665): A r2 >= 1e6,
@@ -542,6 +794,7 @@ void launch_persistent(int ppt, unsigned grid, hipStream_t s, const VArgs *specs
                        const int64_t *item_start, int nspec, unsigned long long *counter) {
     dim3 g(grid), b(kThreads);
     switch (ppt) {
+        case 0: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 0>), g, b, 0, s, specs, item_start, nspec, counter); break;
         case 2: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 2>), g, b, 0, s, specs, item_start, nspec, counter); break;
         case 4: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 4>), g, b, 0, s, specs, item_start, nspec, counter); break;
         case 8: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 8>), g, b, 0, s, specs, item_start, nspec, counter); break;
@@ -550,7 +803,7 @@ void launch_persistent(int ppt, unsigned grid, hipStream_t s, const VArgs *specs
 }
 
 int check_state(const hk_line_state *st, const char *fn) {
-    HK_CHECK_ARG(st->nlines == 0 || (st->nu && st->S && st->ialphaD && st->y && st->hw &&
+    HK_CHECK_ARG(st->nlines == 0 || (st->nu && st->S && st->ialphaD && st->y && st->hw && st->ey &&
                                      st->blk_max && st->radius && st->key),
                  "%s: line state not initialised by hk_line_params", fn);
     return HK_OK;
@@ -565,12 +818,13 @@ int check_grid(const hk_grid *g, const char *fn) {
 
 }  // namespace
 
+// Workspace: the main pass's tile table (tiles of >= 512 points) followed
+// by the line-core pass's (tiles of kCoreW points).
 extern "C" size_t hk_voigt_workspace_bytes(const hk_grid *grid, const hk_tune *tune) {
     if (!grid || grid->nx <= 0) return 0;
-    int64_t w = 256;   // smallest tile
     (void)tune;
-    int64_t ntiles = (grid->nx + w - 1) / w;
-    return (size_t)ntiles * sizeof(TileInfo) + 256;
+    const int64_t ncore = (grid->nx + kCoreW - 1) / kCoreW;
+    return align_up((size_t)ncore * sizeof(TileInfo), 256) * 2 + 256;
 }
 
 extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid, double kmin,
@@ -590,48 +844,81 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
     }
     const int W = kThreads * ppt;
     const int64_t ntiles = (grid->nx + W - 1) / W;
+    const int64_t ncore = (grid->nx + kCoreW - 1) / kCoreW;
     TileInfo *tiles = (TileInfo *)align_up((size_t)ws, 256);
+    TileInfo *ctiles = (TileInfo *)((char *)tiles + align_up((size_t)ncore * sizeof(TileInfo), 256));
+    const int scale = mode == HK_VOIGT_FP32C ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s,
                        st->key, st->nlines, st->radius, st->blk_max, grid->numin, grid->dnu,
-                       grid->nx, W, ntiles, mode == HK_VOIGT_FP32C ? 1 : 0, tiles);
+                       grid->nx, W, ntiles, scale, tiles);
+    HK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((ncore + 3) / 4)), dim3(256), 0, s,
+                       st->key, st->nlines, st->radius + 1, st->blk_max, grid->numin, grid->dnu,
+                       grid->nx, kCoreW, ncore, scale, ctiles);
     HK_LAUNCH_CHECK();
     VArgs a;
-    a.nu = st->nu; a.S = st->S; a.ia = st->ialphaD; a.y = st->y; a.hw = st->hw;
+    a.nu = st->nu; a.S = st->S; a.ia = st->ialphaD; a.y = st->y; a.hw = st->hw; a.ey = st->ey;
+    a.key = st->key; a.blk_max = st->blk_max; a.radius = st->radius; a.nl = st->nlines;
     a.tiles = tiles; a.numin = grid->numin; a.dnu = grid->dnu; a.nx = grid->nx;
     a.kmin = kmin; a.accumulate = accumulate; a.K = K;
     if (mode == HK_VOIGT_FP64) launch_tiles<HK_VOIGT_FP64>(ppt, ntiles, s, a);
     else launch_tiles<HK_VOIGT_FP32C>(ppt, ntiles, s, a);
     HK_LAUNCH_CHECK();
+    a.tiles = ctiles;
+    if (mode == HK_VOIGT_FP64)
+        hipLaunchKernelGGL(voigt_core_kernel<HK_VOIGT_FP64>, dim3((unsigned)ncore), dim3(kThreads),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(voigt_core_kernel<HK_VOIGT_FP32C>, dim3((unsigned)ncore), dim3(kThreads),
+                           0, s, a);
+    HK_LAUNCH_CHECK();
     return HK_OK;
 }
 
 // ---------------------------------------------------------------------------
-// Batched: workspace = [VArgs x nspec | item_start x (nspec+1) | counter |
-//                       tile tables of every spectrum]
+// Batched: workspace = [VArgs x nspec, main pass | VArgs x nspec, core pass |
+//   item_start x (nspec+1), main | item_start x (nspec+1), core | 2 queue
+//   counters | per spectrum: main tile table, core tile table]
+// hk_voigt_batched_prepare writes the tables (one synchronous upload);
+// hk_voigt_batched_launch then issues kernels only, so a caller can capture
+// it in a graph and replay it.
 // ---------------------------------------------------------------------------
 namespace {
 struct BatchLayout {
-    size_t off_args, off_items, off_counter, off_tiles, total;
-    std::vector<size_t> tile_off;
-    std::vector<int64_t> ntiles;
+    size_t off_args, off_cargs, off_items, off_citems, off_counter, total;
+    std::vector<size_t> tile_off, ctile_off;
+    std::vector<int64_t> ntiles, ncore;
 };
 
 BatchLayout batch_layout(int nspec, const hk_spectrum *spec, int W) {
     BatchLayout L;
     size_t o = 0;
     L.off_args = o; o = align_up(o + sizeof(VArgs) * nspec, 256);
+    L.off_cargs = o; o = align_up(o + sizeof(VArgs) * nspec, 256);
     L.off_items = o; o = align_up(o + sizeof(int64_t) * (nspec + 1), 256);
+    L.off_citems = o; o = align_up(o + sizeof(int64_t) * (nspec + 1), 256);
     L.off_counter = o; o = align_up(o + 64, 256);
-    L.off_tiles = o;
     for (int i = 0; i < nspec; ++i) {
-        int64_t nt = (spec[i].grid.nx + W - 1) / W;
+        const int64_t nt = (spec[i].grid.nx + W - 1) / W;
+        const int64_t nc = (spec[i].grid.nx + kCoreW - 1) / kCoreW;
         L.ntiles.push_back(nt);
+        L.ncore.push_back(nc);
         L.tile_off.push_back(o);
         o = align_up(o + sizeof(TileInfo) * nt, 256);
+        L.ctile_off.push_back(o);
+        o = align_up(o + sizeof(TileInfo) * nc, 256);
     }
     L.total = o + 256;
     return L;
+}
+
+unsigned persistent_grid(int64_t items) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    return (unsigned)std::min<int64_t>(items, (int64_t)ncu * 8);
 }
 }  // namespace
 
@@ -639,76 +926,126 @@ extern "C" size_t hk_voigt_batched_workspace_bytes(int nspec, const hk_spectrum 
                                                    const hk_tune *tune) {
     if (nspec <= 0 || !spec) return 0;
     (void)tune;
-    return batch_layout(nspec, spec, 256).total;   // smallest tile: largest tables
+    return batch_layout(nspec, spec, 512).total;   // smallest main tile: largest tables
 }
 
-extern "C" int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode, const hk_tune *tune,
-                                void *ws, size_t ws_bytes, hk_stream_t stream) {
-    HK_CHECK_ARG(nspec > 0 && spec && ws, "hk_voigt_batched: bad arguments");
-    HK_CHECK_ARG(mode == HK_VOIGT_FP64 || mode == HK_VOIGT_FP32C, "hk_voigt_batched: mode %d", mode);
+extern "C" int hk_voigt_batched_prepare(int nspec, const hk_spectrum *spec, int mode,
+                                        const hk_tune *tune, void *ws, size_t ws_bytes,
+                                        hk_voigt_batch *batch, hk_stream_t stream) {
+    HK_CHECK_ARG(nspec > 0 && spec && ws && batch, "hk_voigt_batched_prepare: bad arguments");
+    HK_CHECK_ARG(mode == HK_VOIGT_FP64 || mode == HK_VOIGT_FP32C,
+                 "hk_voigt_batched_prepare: mode %d", mode);
     const int ppt = pick_ppt(mode, tune);
-    HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_batched: tile_points must be 512, 1024, 2048 or 4096");
+    HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_batched_prepare: tile_points must be 512, 1024, 2048 or 4096");
     for (int i = 0; i < nspec; ++i) {
-        if (int e = check_state(&spec[i].st, "hk_voigt_batched")) return e;
-        if (int e = check_grid(&spec[i].grid, "hk_voigt_batched")) return e;
-        HK_CHECK_ARG(spec[i].K, "hk_voigt_batched: spectrum %d has no K", i);
+        if (int e = check_state(&spec[i].st, "hk_voigt_batched_prepare")) return e;
+        if (int e = check_grid(&spec[i].grid, "hk_voigt_batched_prepare")) return e;
+        HK_CHECK_ARG(spec[i].K, "hk_voigt_batched_prepare: spectrum %d has no K", i);
     }
     if (ws_bytes < hk_voigt_batched_workspace_bytes(nspec, spec, tune)) {
-        set_error("hk_voigt_batched: workspace %zu bytes, %zu needed", ws_bytes,
+        set_error("hk_voigt_batched_prepare: workspace %zu bytes, %zu needed", ws_bytes,
                   hk_voigt_batched_workspace_bytes(nspec, spec, tune));
         return HK_ERR_WORKSPACE;
     }
     const int W = kThreads * ppt;
     char *base = (char *)align_up((size_t)ws, 256);
     BatchLayout L = batch_layout(nspec, spec, W);
-    // Host tables: spectra ordered by line count (largest first) so the
-    // heaviest tiles are dequeued first.
+    // Spectra ordered by line count (largest first) so the heaviest tiles are
+    // dequeued first.
     std::vector<int> order(nspec);
     for (int i = 0; i < nspec; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return spec[x].st.nlines > spec[y].st.nlines;
     });
-    std::vector<VArgs> args(nspec);
-    std::vector<int64_t> items(nspec + 1, 0);
-    hipStream_t s = (hipStream_t)stream;
+    std::vector<VArgs> args(nspec), cargs(nspec);
+    std::vector<int64_t> items(nspec + 1, 0), citems(nspec + 1, 0);
     for (int q = 0; q < nspec; ++q) {
         const int i = order[q];
         const hk_spectrum &sp = spec[i];
-        TileInfo *tiles = (TileInfo *)(base + L.tile_off[i]);
-        hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((L.ntiles[i] + 3) / 4)), dim3(256),
-                           0, s, sp.st.key, sp.st.nlines, sp.st.radius, sp.st.blk_max,
-                           sp.grid.numin, sp.grid.dnu, sp.grid.nx, W, L.ntiles[i],
-                           mode == HK_VOIGT_FP32C ? 1 : 0, tiles);
-        HK_LAUNCH_CHECK();
         VArgs &a = args[q];
         a.nu = sp.st.nu; a.S = sp.st.S; a.ia = sp.st.ialphaD; a.y = sp.st.y; a.hw = sp.st.hw;
-        a.tiles = tiles; a.numin = sp.grid.numin; a.dnu = sp.grid.dnu; a.nx = sp.grid.nx;
+        a.ey = sp.st.ey;
+        a.key = sp.st.key; a.blk_max = sp.st.blk_max; a.radius = sp.st.radius;
+        a.nl = sp.st.nlines;
+        a.tiles = (TileInfo *)(base + L.tile_off[i]);
+        a.numin = sp.grid.numin; a.dnu = sp.grid.dnu; a.nx = sp.grid.nx;
         a.kmin = sp.kmin; a.accumulate = sp.accumulate; a.K = sp.K;
+        cargs[q] = a;
+        cargs[q].radius = sp.st.radius + 1;
+        cargs[q].tiles = (TileInfo *)(base + L.ctile_off[i]);
         items[q + 1] = items[q] + L.ntiles[i];
+        citems[q + 1] = citems[q] + L.ncore[i];
     }
+    hipStream_t s = (hipStream_t)stream;
     HK_HIP(hipMemcpyAsync(base + L.off_args, args.data(), sizeof(VArgs) * nspec,
+                          hipMemcpyHostToDevice, s));
+    HK_HIP(hipMemcpyAsync(base + L.off_cargs, cargs.data(), sizeof(VArgs) * nspec,
                           hipMemcpyHostToDevice, s));
     HK_HIP(hipMemcpyAsync(base + L.off_items, items.data(), sizeof(int64_t) * (nspec + 1),
                           hipMemcpyHostToDevice, s));
-    unsigned long long *counter = (unsigned long long *)(base + L.off_counter);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
-    HK_LAUNCH_CHECK();
-    int dev = 0, ncu = 0;
-    HK_HIP(hipGetDevice(&dev));
-    HK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int64_t total = items[nspec];
-    unsigned grid = (unsigned)std::min<int64_t>(total, (int64_t)ncu * 8);
-    if (grid == 0) return HK_OK;
-    if (mode == HK_VOIGT_FP64)
-        launch_persistent<HK_VOIGT_FP64>(ppt, grid, s, (const VArgs *)(base + L.off_args),
-                                         (const int64_t *)(base + L.off_items), nspec, counter);
-    else
-        launch_persistent<HK_VOIGT_FP32C>(ppt, grid, s, (const VArgs *)(base + L.off_args),
-                                          (const int64_t *)(base + L.off_items), nspec, counter);
-    HK_LAUNCH_CHECK();
+    HK_HIP(hipMemcpyAsync(base + L.off_citems, citems.data(), sizeof(int64_t) * (nspec + 1),
+                          hipMemcpyHostToDevice, s));
     // the host tables must outlive the asynchronous copies
     HK_HIP(hipStreamSynchronize(s));
+    batch->workspace = base;
+    batch->nspec = nspec;
+    batch->mode = mode;
+    batch->tile_points = W;
+    batch->pad = 0;
+    batch->items_main = items[nspec];
+    batch->items_core = citems[nspec];
+    batch->off_args = L.off_args; batch->off_cargs = L.off_cargs;
+    batch->off_items = L.off_items; batch->off_citems = L.off_citems;
+    batch->off_counter = L.off_counter;
     return HK_OK;
+}
+
+extern "C" int hk_voigt_batched_launch(const hk_voigt_batch *batch, hk_stream_t stream) {
+    HK_CHECK_ARG(batch && batch->workspace && batch->nspec > 0,
+                 "hk_voigt_batched_launch: batch not prepared");
+    const int ppt = batch->tile_points / kThreads;
+    HK_CHECK_ARG(ppt_ok(ppt) && (batch->mode == HK_VOIGT_FP64 || batch->mode == HK_VOIGT_FP32C),
+                 "hk_voigt_batched_launch: batch not prepared");
+    char *base = (char *)batch->workspace;
+    const VArgs *args = (const VArgs *)(base + batch->off_args);
+    const VArgs *cargs = (const VArgs *)(base + batch->off_cargs);
+    const int64_t *items = (const int64_t *)(base + batch->off_items);
+    const int64_t *citems = (const int64_t *)(base + batch->off_citems);
+    unsigned long long *counter = (unsigned long long *)(base + batch->off_counter);
+    const int nspec = batch->nspec;
+    const int scale = batch->mode == HK_VOIGT_FP32C ? 1 : 0;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counter);
+    HK_LAUNCH_CHECK();
+    if (batch->items_main > 0) {
+        hipLaunchKernelGGL(batched_ranges_kernel, dim3((unsigned)((batch->items_main + 3) / 4)),
+                           dim3(256), 0, s, args, items, nspec, batch->tile_points, scale);
+        HK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(batched_ranges_kernel, dim3((unsigned)((batch->items_core + 3) / 4)),
+                           dim3(256), 0, s, cargs, citems, nspec, kCoreW, scale);
+        HK_LAUNCH_CHECK();
+        const unsigned gm = persistent_grid(batch->items_main);
+        const unsigned gc = persistent_grid(batch->items_core);
+        if (batch->mode == HK_VOIGT_FP64) {
+            launch_persistent<HK_VOIGT_FP64>(ppt, gm, s, args, items, nspec, counter);
+            HK_LAUNCH_CHECK();
+            launch_persistent<HK_VOIGT_FP64>(0, gc, s, cargs, citems, nspec, counter + 1);
+        } else {
+            launch_persistent<HK_VOIGT_FP32C>(ppt, gm, s, args, items, nspec, counter);
+            HK_LAUNCH_CHECK();
+            launch_persistent<HK_VOIGT_FP32C>(0, gc, s, cargs, citems, nspec, counter + 1);
+        }
+        HK_LAUNCH_CHECK();
+    }
+    return HK_OK;
+}
+
+extern "C" int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode, const hk_tune *tune,
+                                void *ws, size_t ws_bytes, hk_stream_t stream) {
+    hk_voigt_batch b;
+    if (int e = hk_voigt_batched_prepare(nspec, spec, mode, tune, ws, ws_bytes, &b, stream))
+        return e;
+    return hk_voigt_batched_launch(&b, stream);
 }
 
 extern "C" int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,

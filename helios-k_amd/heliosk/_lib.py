@@ -44,7 +44,7 @@ class Grid(ctypes.Structure):
 
 class LineState(ctypes.Structure):
     _fields_ = [("nlines", c_i64), ("nu", c_vp), ("S", c_vp), ("ialphaD", c_vp), ("y", c_vp),
-                ("hw", c_vp), ("blk_max", c_vp), ("radius", c_vp), ("key", c_vp)]
+                ("hw", c_vp), ("ey", c_vp), ("blk_max", c_vp), ("radius", c_vp), ("key", c_vp)]
 
 
 class Tune(ctypes.Structure):
@@ -54,6 +54,13 @@ class Tune(ctypes.Structure):
 class Spectrum(ctypes.Structure):
     _fields_ = [("st", LineState), ("grid", Grid), ("kmin", c_d), ("accumulate", c_int),
                 ("K", c_vp)]
+
+
+class VoigtBatch(ctypes.Structure):
+    _fields_ = [("workspace", c_vp), ("nspec", c_int), ("mode", c_int), ("tile_points", c_int),
+                ("pad", c_int), ("items_main", c_i64), ("items_core", c_i64),
+                ("off_args", c_size), ("off_cargs", c_size), ("off_items", c_size),
+                ("off_citems", c_size), ("off_counter", c_size)]
 
 
 _lib = None
@@ -85,6 +92,9 @@ def lib():
         L.hk_voigt_batched_workspace_bytes.restype = c_size
         L.hk_voigt_batched_workspace_bytes.argtypes = [c_int, P(Spectrum), P(Tune)]
         L.hk_voigt_batched.argtypes = [c_int, P(Spectrum), c_int, P(Tune), c_vp, c_size, c_vp]
+        L.hk_voigt_batched_prepare.argtypes = [c_int, P(Spectrum), c_int, P(Tune), c_vp, c_size,
+                                               P(VoigtBatch), c_vp]
+        L.hk_voigt_batched_launch.argtypes = [P(VoigtBatch), c_vp]
         L.hk_ktable_workspace_bytes.restype = c_size
         L.hk_ktable_workspace_bytes.argtypes = [c_i64, c_int, c_int, c_int]
         L.hk_ktable_prepare.argtypes = [P(Grid), c_d, c_int, c_int, c_vp, c_size]
@@ -112,7 +122,8 @@ def check(rc):
 # Every symbol include/heliosk.h declares.
 EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state_init",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
-            "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_ktable_workspace_bytes",
+            "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",
+            "hk_voigt_batched_launch", "hk_ktable_workspace_bytes",
             "hk_ktable_prepare", "hk_ktable", "hk_ktable_reconstruct", "hk_ktable_plan_host",
             "hk_voigt_adjoint_workspace_bytes", "hk_voigt_adjoint", "hk_bin_segments",
             "hk_synth_lines"]

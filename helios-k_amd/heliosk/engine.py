@@ -16,7 +16,8 @@ from ._lib import (HK_VOIGT_FP32C, HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineSta
 
 NA = 6.0221412927e23          # define.h:42
 
-__all__ = ["Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "ktable",
+__all__ = ["Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
+           "ktable",
            "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
 
@@ -145,18 +146,44 @@ def pair_count(state, grid, stream=None):
     return [int(v) for v in out.cpu()]
 
 
-def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
-    """items: list of (State, Grid, kmin, K tensor).  One persistent launch."""
-    n = len(items)
-    arr = (Spectrum * n)()
+def _spectra(items):
+    arr = (Spectrum * len(items))()
     for i, (st, g, kmin, K) in enumerate(items):
         arr[i].st = st.st; arr[i].grid = g; arr[i].kmin = float(kmin)
         arr[i].accumulate = 0; arr[i].K = K.data_ptr()
+    return arr
+
+
+def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
+    """items: list of (State, Grid, kmin, K tensor).  One persistent launch
+    per pass over all spectra."""
+    n = len(items)
+    arr = _spectra(items)
     tune = _tune(tile_points)
     nbytes = lib().hk_voigt_batched_workspace_bytes(n, arr, ctypes.byref(tune))
     ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
     check(lib().hk_voigt_batched(n, arr, int(mode), ctypes.byref(tune), _ptr(ws), nbytes,
                                  _stream(stream)))
+
+
+class VoigtBatch:
+    """A prepared batch of spectra: launch() issues kernels only, so it can be
+    captured in a torch.cuda.CUDAGraph (a hipGraph) and replayed."""
+
+    def __init__(self, items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
+        self.items = list(items)            # keeps the states and K tensors alive
+        n = len(self.items)
+        arr = _spectra(self.items)
+        tune = _tune(tile_points)
+        nbytes = lib().hk_voigt_batched_workspace_bytes(n, arr, ctypes.byref(tune))
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+        self.batch = _lib.VoigtBatch()
+        check(lib().hk_voigt_batched_prepare(n, arr, int(mode), ctypes.byref(tune),
+                                             _ptr(self.ws), nbytes, ctypes.byref(self.batch),
+                                             _stream(stream)))
+
+    def launch(self, stream=None):
+        check(lib().hk_voigt_batched_launch(ctypes.byref(self.batch), _stream(stream)))
 
 
 class KTable:

@@ -1,0 +1,153 @@
+"""Readers and writers of the driver's file formats.
+
+Writers produce inputs in the reference's layouts, so synthetic line lists
+run through the real driver path:
+  * HITRAN/HITEMP binary line files: 68-byte records, char cid[4] then
+    nu, S, EL, A, delta, gamma_air, gamma_self, n as float64 (host.h:605-615);
+  * <species>.param files (ISO.h:16-219);
+  * partition-function tables, two columns T Q (host.h:9-73);
+  * param.dat (host.h:75-300).
+Readers parse the driver's outputs: Out_<name>.bin (float32 K, P-major,
+heliosk.cu:1889-1911), Out_<name>.dat (heliosk.cu:1839-1861), the k-table
+rows of Out_<name>_cbin.dat (heliosk.cu:2166-2181) and the per-bin K of
+Out_<name>_bin.dat (heliosk.cu:2245-2283).
+"""
+import os
+
+import numpy as np
+
+HITRAN_RECORD = np.dtype([("cid", "S4"), ("nu", "<f8"), ("S", "<f8"), ("EL", "<f8"), ("A", "<f8"),
+                          ("delta", "<f8"), ("gair", "<f8"), ("gself", "<f8"), ("n", "<f8")])
+assert HITRAN_RECORD.itemsize == 68
+
+# param.dat keys in the reference's order (param.dat:1-35)
+PARAM_DEFAULTS = [
+    ("name", "i"), ("T", 1500.0), ("P", 1.0), ("PFile", "-"), ("Species Name", "species"),
+    ("SpeciesFile", "-"), ("ciaSystem", "-"), ("pathToData", ""), ("numin", 0.0), ("numax", 1000.0),
+    ("dnu", 0.01), ("Nnu per bin", 0), ("cutMode", 0), ("cut", 25.0), ("doResampling", 0),
+    ("nC", 20), ("doTransmission", 0), ("nTr", 1000), ("dTr", 0.05), ("doStoreFullK", 1),
+    ("pathToK", ""), ("doStoreSK", 0), ("nbins", 1), ("binsFile", "-"), ("OutputEdgesFile", "-"),
+    ("kmin", 0.0), ("qalphaL", 1.0), ("gammaF", 1.0), ("doMean", 0), ("Units", 0),
+    ("ReplaceFiles", 1), ("profile", 1), ("subLorentzianfile", "-"), ("removePlinth", 0),
+    ("doTuning", 1),
+]
+
+
+def cid_bytes(cid):
+    """Isotopologue id as stored in a record: "%3s" plus the terminating NUL."""
+    return ("%3s" % cid).encode()[:3] + b"\0"
+
+
+def write_hitran_bin(path, cid, nu, S, EL, A, delta, gair, gself, n):
+    nl = len(nu)
+    rec = np.zeros(nl, dtype=HITRAN_RECORD)
+    rec["cid"] = [cid_bytes(c) for c in (cid if np.ndim(cid) else [cid] * nl)]
+    for k, v in (("nu", nu), ("S", S), ("EL", EL), ("A", A), ("delta", delta), ("gair", gair),
+                 ("gself", gself), ("n", n)):
+        rec[k] = v
+    rec.tofile(path)
+
+
+def read_hitran_bin(path):
+    return np.fromfile(path, dtype=HITRAN_RECORD)
+
+
+def write_partition(path, T, Q):
+    with open(path, "w") as f:
+        for t, q in zip(T, Q):
+            f.write("%12.4f %20.10f\n" % (t, q))
+
+
+def write_species_param(path, name, isotopologues, nlines, limits, database=0, molecule=1,
+                        npfcol=2):
+    """isotopologues: [(cid, abundance, Q296, g, molar mass, partition file)];
+    nlines: lines per file; limits: nfiles+1 file limits (cm^-1)."""
+    with open(path, "w") as f:
+        f.write("Database = %d\n" % database)
+        f.write("Molecule number = %d\n" % molecule)
+        f.write("Name = %s\n" % name)
+        f.write("Number of Isotopologues = %d\n" % len(isotopologues))
+        f.write("#ID Abundance      Q(296K)   g     Molar Mass(g)  partition file :\n")
+        for cid, ab, q, g, m, qf in isotopologues:
+            f.write(" %s %.10g %.10g %d %.12g %s\n" % (cid, ab, q, g, m, qf))
+        f.write("Number of columns in partition File = %d\n" % npfcol)
+        f.write("Number of line/transition files = %d\n" % len(nlines))
+        f.write("Number of lines per file :\n")
+        for k in nlines:
+            f.write("%d\n" % k)
+        f.write("Line file limits :\n")
+        for v in limits:
+            f.write("%d\n" % v)
+        f.write("#ExoMol :\n")
+        f.write("Number of states = 0\n")
+        f.write("Number of columns in transition files = 0\n")
+        f.write("Default value of Lorentzian half-width for all lines = 0\n")
+        f.write("Default value of temperature exponent for all lines = 0\n")
+        f.write("Version = 0\n")
+
+
+def write_param_dat(path, **overrides):
+    """param.dat with the reference's keys; overrides use the key text with
+    spaces replaced by underscores (e.g. Species_Name="01_hit16")."""
+    vals = dict(PARAM_DEFAULTS)
+    for k, v in overrides.items():
+        key = k.replace("_", " ") if k.replace("_", " ") in vals else k
+        if key not in vals and key != "precision":
+            raise KeyError(k)
+        vals[key] = v
+    with open(path, "w") as f:
+        for k, _ in PARAM_DEFAULTS:
+            f.write("%s = %s\n" % (k, vals[k]))
+        if "precision" in vals:
+            f.write("precision = %s\n" % vals["precision"])
+
+
+def read_out_bin(path, nx):
+    K = np.fromfile(path, dtype="<f4")
+    return K.reshape(-1, nx)
+
+
+def read_out_dat(path):
+    """Blocks of "x K [T P]" rows separated by blank lines -> list of arrays."""
+    blocks, cur = [], []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if s:
+                cur.append([float(v) for v in s])
+            elif cur:
+                blocks.append(np.array(cur))
+                cur = []
+    if cur:
+        blocks.append(np.array(cur))
+    return blocks
+
+
+def read_cbin(path, nC):
+    """k-table rows: kmin, ystart, nC coefficients [, T, P]."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if s:
+                rows.append([float(v) for v in s])
+    out = np.array(rows)
+    return {"kmin": out[:, 0], "ystart": out[:, 1], "coeff": out[:, 2:2 + nC],
+            "TP": out[:, 2 + nC:] if out.shape[1] > 2 + nC else None}
+
+
+def species_files(dirpath, name, lines, isotopologues, T_table, Q_tables, limits=None):
+    """Write <name>.param, the partition tables and <name>.bin for one
+    species.  lines: dict of arrays nu, S, EL, A, delta, gair, gself, n and
+    per-line cid; isotopologues: [(cid, abundance, Q296, g, molar mass)]."""
+    isos = []
+    for k, (cid, ab, q, g, m) in enumerate(isotopologues):
+        qf = "q_%s_%d.txt" % (name, k)
+        write_partition(os.path.join(dirpath, qf), T_table, Q_tables[k])
+        isos.append((cid, ab, q, g, m, qf))
+    nl = len(lines["nu"])
+    lim = limits if limits is not None else [0, int(np.ceil(np.max(lines["nu"]))) + 1 if nl else 1]
+    write_species_param(os.path.join(dirpath, name + ".param"), name, isos, [nl], lim)
+    write_hitran_bin(os.path.join(dirpath, name + ".bin"), lines["cid"], lines["nu"], lines["S"],
+                     lines["EL"], lines["A"], lines["delta"], lines["gair"], lines["gself"],
+                     lines["n"])

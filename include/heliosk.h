@@ -100,8 +100,10 @@ typedef struct {
     double *ialphaD;    /* 1 / alpha_D */
     double *y;          /* alpha_L / alpha_D */
     double *hw;         /* cut half width [cm^-1]; < 0 marks an empty line */
+    double *ey;         /* erfcx(y), used by the line-core series */
     double *blk_max;    /* max S over each block of 256 lines, ceil(n/256) */
-    double *radius;     /* 1 value: max_i (hw_i + |nu_i - key_i|) */
+    double *radius;     /* radius[0] = max_i (hw_i + |nu_i - key_i|), radius[1] =
+                           max_i (min(hw_i, 10 alpha_D,i) + |nu_i - key_i|) */
     const double *key;  /* sorted search key (the raw nu), set by hk_line_params */
 } hk_line_state;
 
@@ -115,7 +117,7 @@ const char *hk_last_error(void);
 const char *hk_version(void);
 
 /* Bytes of device memory an hk_line_state of n lines needs (one
- * allocation holding nu, S, ialphaD, y, hw, blk_max and radius). */
+ * allocation holding nu, S, ialphaD, y, hw, ey, blk_max and radius). */
 size_t hk_line_state_bytes(int64_t nlines);
 /* Lay out an hk_line_state inside caller memory of hk_line_state_bytes. */
 int hk_line_state_init(hk_line_state *st, int64_t nlines, void *mem, size_t bytes);
@@ -156,6 +158,24 @@ size_t hk_voigt_batched_workspace_bytes(int nspec, const hk_spectrum *spec,
 int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode,
                      const hk_tune *tune, void *workspace,
                      size_t workspace_bytes, hk_stream_t stream);
+
+/* The same in two steps.  _prepare uploads the batch tables into the
+ * workspace (returns after the upload) and fills *batch; _launch issues
+ * kernels only (no allocation, copy or host synchronisation), so it can be
+ * captured in a hipGraph and replayed: each replay recomputes every
+ * spectrum from the current contents of the line states. */
+typedef struct {
+    void *workspace;
+    int nspec, mode, tile_points, pad;
+    int64_t items_main, items_core;
+    size_t off_args, off_cargs, off_items, off_citems, off_counter;
+} hk_voigt_batch;
+
+int hk_voigt_batched_prepare(int nspec, const hk_spectrum *spec, int mode,
+                             const hk_tune *tune, void *workspace,
+                             size_t workspace_bytes, hk_voigt_batch *batch,
+                             hk_stream_t stream);
+int hk_voigt_batched_launch(const hk_voigt_batch *batch, hk_stream_t stream);
 
 /* k-distribution tables.  K holds nspec spectra of grid->nx points; bins are
  * equal in wavenumber over [numin, numax) and hold Nxb = nx/nbins points.

@@ -1,0 +1,82 @@
+"""CPU checks of the heliosk driver's front end: the file writers round-trip,
+and param.dat / console parsing and validation reject what the reference
+rejects (with its messages and its exit status 0) before any device is
+touched."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from heliosk import files as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "helios-k_amd", "bin", "heliosk")
+
+
+def run_driver(cwd, *args):
+    if not os.path.exists(DRIVER):
+        pytest.fail("driver not built: %s (run __graft_entry__.build())" % DRIVER)
+    r = subprocess.run([DRIVER, *args], cwd=cwd, capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout
+
+
+def test_hitran_record_roundtrip(tmp_path):
+    rng = np.random.default_rng(3)
+    n = 50
+    d = dict(nu=np.sort(rng.uniform(0, 100, n)), S=10 ** rng.uniform(-25, -19, n),
+             EL=rng.uniform(0, 3000, n), A=rng.uniform(0, 1, n), delta=rng.uniform(-0.01, 0.01, n),
+             gair=rng.uniform(0.01, 0.1, n), gself=rng.uniform(0.1, 0.5, n), n=rng.uniform(0.3, 0.8, n))
+    cid = ["11" if k % 3 else "12" for k in range(n)]
+    p = tmp_path / "x.bin"
+    F.write_hitran_bin(str(p), cid, **d)
+    assert os.path.getsize(p) == 68 * n
+    rec = F.read_hitran_bin(str(p))
+    for k, v in d.items():
+        np.testing.assert_array_equal(rec[k], v)
+    assert rec["cid"][1] == b" 11" and rec["cid"][0] == b" 12"
+    raw = open(p, "rb").read()
+    assert raw[:4] == b" 12\0"
+
+
+def _setup(tmp_path, **kw):
+    F.write_param_dat(str(tmp_path / "param.dat"), **kw)
+
+
+@pytest.mark.parametrize("kw, msg", [
+    (dict(profile=2), "Error: profile 2 is not supported"),
+    (dict(numin=0.0, numax=10.0, dnu=0.01, nbins=7), "Error: range cannot be divided evenly in bins."),
+    (dict(ciaSystem="H2-H2"), "Error: CIA systems are not supported"),
+    (dict(doTransmission=1), "Error: transmission functions are not supported"),
+    (dict(binsFile="bins.dat"), "Error: bins files are not supported"),
+])
+def test_driver_rejects_like_the_reference(tmp_path, kw, msg):
+    _setup(tmp_path, **kw)
+    rc, out = run_driver(str(tmp_path))
+    assert rc == 0                         # the reference exits 0 after "Error: ..."
+    assert msg in out, out
+
+
+def test_driver_undefined_key(tmp_path):
+    (tmp_path / "param.dat").write_text("name = a\nT = 300\nnot a key = 3\n")
+    rc, out = run_driver(str(tmp_path))
+    assert rc == 0 and "Undefined line in param.dat file: line 2" in out, out
+
+
+def test_driver_console_overrides(tmp_path):
+    _setup(tmp_path, numin=0.0, numax=10.0, dnu=0.01, nbins=5)
+    rc, out = run_driver(str(tmp_path), "-nbins", "7")
+    assert "range cannot be divided evenly in bins. 1000 7" in out, out
+    rc, out = run_driver(str(tmp_path), "-bogus", "1")
+    assert "Error: Console arguments not valid!" in out, out
+    rc, out = run_driver(str(tmp_path), "-dnu", "0.003")
+    assert "range cannot be divided evenly in bins. 3333 5" in out, out
+
+
+def test_driver_empty_path_values(tmp_path):
+    # "pathToData =" and "pathToK =" with no value: the reader takes the next
+    # key's value (host.h:128-138, 197-205)
+    _setup(tmp_path, numin=2.0, numax=10.0, dnu=0.01, nbins=3)
+    assert "pathToData = \nnumin = 2.0" in (tmp_path / "param.dat").read_text()
+    rc, out = run_driver(str(tmp_path))
+    assert "range cannot be divided evenly in bins. 800 3" in out, out

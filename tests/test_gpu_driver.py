@@ -1,0 +1,176 @@
+"""The heliosk driver end to end on the GPU: synthetic HITRAN-format inputs
+(param.dat, <species>.param, partition tables, 68-byte line records) go
+through the real front end, and its output files are checked against the
+FP64 oracle fed with the same lines (readFile's strength scaling and
+partition interpolation restated here from host.h:9-73, 598-649)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from heliosk import files as F
+from helpers import NA, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "helios-k_amd", "bin", "heliosk")
+
+T_TABLE = np.arange(1.0, 3001.0, 1.0)
+
+
+def q_table(k):
+    return 1.0 + (0.05 + 0.01 * k) * T_TABLE ** 1.5
+
+
+def q_at(T, Q):
+    """Linear interpolation between the bracketing rows T0 < T <= T1."""
+    j = int(np.searchsorted(T_TABLE, T, side="left"))
+    if j == 0:
+        return Q[0]
+    t = (T - T_TABLE[j - 1]) / (T_TABLE[j] - T_TABLE[j - 1])
+    return (Q[j] - Q[j - 1]) * t + Q[j - 1]
+
+
+def make_species(tmp, name, cids, masses, abund, n, seed, numin, numax):
+    rng = np.random.default_rng(seed)
+    lines = dict(nu=rng.uniform(numin, numax, n), S=10 ** rng.uniform(-25, -19, n),
+                 EL=rng.uniform(0, 3000, n), A=rng.uniform(0, 2e7, n) * (rng.uniform(size=n) < 0.2),
+                 delta=rng.uniform(-0.01, 0.01, n), gair=rng.uniform(0.01, 0.1, n),
+                 gself=rng.uniform(0.1, 0.4, n), n=rng.uniform(0.3, 0.8, n))
+    iso = rng.integers(0, len(cids), n)
+    lines["cid"] = [cids[k] for k in iso]
+    isos = [(c, a, 170.0 + 10 * k, 1, m) for k, (c, a, m) in enumerate(zip(cids, abund, masses))]
+    F.species_files(str(tmp), name, lines, isos, T_TABLE, [q_table(k) for k in range(len(cids))],
+                    limits=[int(numin) - 1, int(numax) + 1])
+    return dict(lines=lines, iso=iso, masses=np.array(masses), abund=np.array(abund),
+                Qtab=[q_table(k) for k in range(len(cids))])
+
+
+def oracle_K(species, sscale, meanMass, T, P, q, gammaF, cut, numin, dnu, nx, kmin, units=0):
+    """Sum over species of the oracle cross section, with readFile's
+    per-isotopologue scaling (host.h:628-644) and S2_kernel semantics."""
+    K = np.full(nx, kmin)
+    for sp, ss in zip(species, sscale):
+        L = sp["lines"]
+        m = sp["masses"]
+        ab = sp["abund"] * (m / meanMass if units == 0 else 1.0)
+        sc = np.full(len(m), ss) * (m / meanMass if units == 0 else 1.0)
+        d = dict(nu=L["nu"], S=L["S"], EL=L["EL"], A=L["A"], delta=L["delta"],
+                 vy=((1.0 - q) * L["gair"] + q * L["gself"]) * gammaF, n=L["n"],
+                 iso=sp["iso"].astype(np.int32), iso_mass=m / NA, iso_ab=ab, iso_sscale=sc,
+                 iso_Q=np.array([q_at(T, Q) for Q in sp["Qtab"]]))
+        order = np.argsort(d["nu"], kind="stable")
+        d = {k: (v[order] if isinstance(v, np.ndarray) and v.shape == order.shape else v)
+             for k, v in d.items()}
+        st = O.line_params(d, T, P, 0, cut)
+        K += O.voigt_points(st, numin, dnu, nx, kmin=0.0)
+    return K
+
+
+def run_driver(cwd, *args):
+    assert os.path.exists(DRIVER), "driver not built"
+    print("driver: %s %s" % (cwd, " ".join(args)), flush=True)
+    r = subprocess.run([DRIVER, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    assert "Error" not in r.stdout, r.stdout
+    return r.stdout
+
+
+NUMIN, NUMAX, DNU, NBINS, NC = 1000.0, 1010.0, 0.001, 5, 16
+
+
+def test_driver_text_outputs_vs_oracle(tmp_path):
+    sp = make_species(tmp_path, "01_tst", ["11", "12"], [18.010565, 20.014811], [0.997317, 0.002],
+                      3000, 1, 995.0, 1015.0)
+    T, P, q, gF, cut = 1234.5, 0.7, 0.3, 1.1, 5.0
+    F.write_param_dat(str(tmp_path / "param.dat"), name="t", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doResampling=1, nC=NC,
+                      doStoreFullK=1, nbins=NBINS, qalphaL=q, gammaF=gF, doStoreSK=1)
+    out = run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    Ko = oracle_K([sp], [1.0], meanMass, T, P, q, gF, cut, NUMIN, DNU, nx, 0.0)
+    blk = F.read_out_dat(str(tmp_path / "Out_t.dat"))
+    assert len(blk) == 1 and blk[0].shape == (nx, 2)
+    np.testing.assert_allclose(blk[0][:, 0], NUMIN + DNU * np.arange(nx), rtol=0, atol=1e-9)
+    Kd = blk[0][:, 1]
+    e = rel_err(Kd, Ko)
+    print("driver K vs oracle %.3e" % e)
+    assert e <= 1e-11
+    # k-table rows against the oracle fit of the driver's own K
+    seg = O.bin_segments(NUMIN, NUMAX, DNU, nx, NBINS)
+    co, no = O.ktable(O.sort_segments(Kd, seg[:NBINS + 1]), nx // NBINS, NBINS, NC, 0.0)
+    cb = F.read_cbin(str(tmp_path / "Out_t_cbin.dat"), NC)
+    Nxb = nx // NBINS
+    np.testing.assert_array_equal(cb["ystart"], np.minimum(no / (Nxb - 1.0), 1.0))
+    ref = np.where((no == Nxb)[:, None], 0.0, co)
+    assert np.max(np.abs(cb["coeff"] - ref)) <= 1e-10 * np.max(np.abs(ref))
+    # K per bin: the reconstruction from the fitted coefficients
+    kb = F.read_out_dat(str(tmp_path / "Out_t_bin.dat"))
+    assert len(kb) == NBINS
+    Ky = O.reconstruct(cb["coeff"], no, Nxb, 0.0)
+    got = np.concatenate([b[:, 1] for b in kb])
+    assert rel_err(got, Ky) <= 1e-12
+    np.testing.assert_allclose(kb[0][:, 0], np.array(["%g" % (j / (Nxb - 1.0)) for j in range(Nxb)],
+                                                      dtype=float))
+    assert "Time for K(x)" in out
+
+
+def test_driver_binary_mixture_pressures_and_kread(tmp_path):
+    s1 = make_species(tmp_path, "01_a", ["11", "12"], [18.010565, 20.014811], [0.997317, 0.002],
+                      2000, 2, 995.0, 1015.0)
+    s2 = make_species(tmp_path, "02_b", ["21"], [43.98983], [0.984204], 1500, 3, 990.0, 1020.0)
+    (tmp_path / "P.txt").write_text("0.7\n3.0\n")
+    (tmp_path / "species.txt").write_text("01_a 0.6\n02_b 0.4\n")
+    T, q, gF, cut = 987.0, 1.0, 1.0, 4.0
+    F.write_param_dat(str(tmp_path / "param.dat"), name="m", T=T, P=1.0, PFile="P.txt",
+                      SpeciesFile="species.txt", numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut,
+                      doResampling=1, nC=NC, doStoreFullK=2, nbins=NBINS, qalphaL=q, gammaF=gF,
+                      Units=1, kmin=1e-48)
+    run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    meanMass = float(0.6 * np.sum(s1["abund"] * s1["masses"]) + 0.4 * np.sum(s2["abund"] * s2["masses"]))
+    unit = meanMass / NA
+    Kf = F.read_out_bin(str(tmp_path / "Out_m.bin"), nx)
+    assert Kf.shape == (2, nx)
+    cb = F.read_cbin(str(tmp_path / "Out_m_cbin.dat"), NC)
+    assert cb["coeff"].shape == (2 * NBINS, NC)
+    np.testing.assert_array_equal(cb["TP"], np.repeat([[T, 0.7], [T, 3.0]], NBINS, axis=0))
+    # Units = 1: kmin is divided by the unit scale and printed so (heliosk.cu:697-704)
+    np.testing.assert_allclose(cb["kmin"], 1e-48 / unit, rtol=1e-15)
+    for iP, P in enumerate((0.7, 3.0)):
+        Ko = oracle_K([s1, s2], [0.6, 0.4], meanMass, T, P, q, gF, cut, NUMIN, DNU, nx,
+                      1e-48 / unit, units=1)
+        e = rel_err(Kf[iP].astype(np.float64), Ko * unit)
+        print("P %g: float32 output vs oracle %.3e" % (P, e))
+        assert e <= 1.2e-7
+    # k-table-only run from the binary cross sections (doStoreFullK = -2)
+    os.rename(tmp_path / "Out_m_cbin.dat", tmp_path / "first_cbin.dat")
+    F.write_param_dat(str(tmp_path / "param.dat"), name="m", T=T, P=1.0, PFile="P.txt",
+                      SpeciesFile="species.txt", numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut,
+                      doResampling=1, nC=NC, doStoreFullK=-2, nbins=NBINS, qalphaL=q, gammaF=gF,
+                      Units=1, kmin=1e-48, pathToK="./")
+    run_driver(str(tmp_path))
+    cb2 = F.read_cbin(str(tmp_path / "Out_m_cbin.dat"), NC)
+    # the same k-tables up to the float32 rounding of the stored K
+    np.testing.assert_array_equal(cb2["ystart"], cb["ystart"])
+    assert np.max(np.abs(cb2["coeff"] - cb["coeff"])) <= 1e-5 * np.max(np.abs(cb["coeff"]))
+
+
+def test_driver_fp32c_precision_key(tmp_path):
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 2500, 5, 995.0, 1015.0)
+    T, P, cut = 1800.0, 2.0, 5.0
+    F.write_param_dat(str(tmp_path / "param.dat"), name="f", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=1, nbins=NBINS,
+                      precision=1)
+    run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    Ko = oracle_K([sp], [1.0], meanMass, T, P, 1.0, 1.0, cut, NUMIN, DNU, nx, 0.0)
+    Kd = F.read_out_dat(str(tmp_path / "Out_f.dat"))[0][:, 1]
+    e = rel_err(Kd, Ko)
+    print("driver FP32C K vs oracle %.3e" % e)
+    assert 0.0 < e <= 1e-5

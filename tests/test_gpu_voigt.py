@@ -71,6 +71,8 @@ VOIGT_CASES = [
     ("low-T-low-P", 100.0, 1e-6, 0, 1.0, 400, (1000.0, 1002.0, 0.0001), (999.5, 1002.5)),
     # many line cores overlap every point and every 256-line LDS stage
     ("dense-cores", 2000.0, 9.87, 0, 5.0, 20000, (10000.0, 10010.0, 0.001), (9995.0, 10015.0)),
+    # each line core spans thousands of points (many line-core tiles)
+    ("wide-cores", 3000.0, 1e-4, 0, 1.0, 100, (1000.0, 1004.0, 0.00001), (999.5, 1004.5)),
 ]
 
 
@@ -149,6 +151,58 @@ def test_voigt_batched_matches_single(E):
         E.voigt_batched([(st, grid, 0.0, K) for st, K in zip(items, outs)], mode=mode)
         for k, K in enumerate(outs):
             np.testing.assert_array_equal(K.cpu().numpy(), refs[2 * k + mi][1])
+
+
+def test_voigt_batch_graph_replay(E):
+    """The prepared batch captured in a graph: every replay equals the eager
+    launch bit for bit, with eager work between replays, and a replay after
+    the line states change recomputes from the new states."""
+    import torch
+    grid = _grid(E, 1000.0, 1010.0, 0.001)
+    sps, sts = [], []
+    for s, n in enumerate((300, 1500)):
+        d = random_lines(n, seed=40 + s, numin=990.0, numax=1020.0)
+        sp = species_from(d)
+        sps.append((sp, d))
+        sts.append(E.line_params(sp, 1200.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=8.0))
+    for mode in (E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C):
+        outs = [torch.empty(grid.nx, dtype=torch.float64, device="cuda") for _ in sts]
+        refs = [E.voigt(st, grid, 1e-30, mode).cpu().numpy() for st in sts]
+        batch = E.VoigtBatch([(st, grid, 1e-30, K) for st, K in zip(sts, outs)], mode=mode)
+        batch.launch()
+        torch.cuda.synchronize()
+        for K, r in zip(outs, refs):
+            np.testing.assert_array_equal(K.cpu().numpy(), r)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            batch.launch()            # warm-up on the capture stream's device
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            batch.launch()
+        for rep in range(3):
+            for K in outs:
+                K.fill_(-1.0)
+            other = E.voigt(sts[0], grid, 0.0, mode)          # eager work between replays
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for K, r in zip(outs, refs):
+                np.testing.assert_array_equal(K.cpu().numpy(), r, err_msg="replay %d" % rep)
+            del other
+        # new (T, P) in the same line-state memory, then replay
+        for (sp, d), st in zip(sps, sts):
+            E.line_params(sp, 400.0, 0.5, Q=d["iso_Q"], cut_mode=0, cut=8.0, state=st)
+        refs2 = [E.voigt(st, grid, 1e-30, mode).cpu().numpy() for st in sts]
+        g.replay()
+        torch.cuda.synchronize()
+        for K, r in zip(outs, refs2):
+            np.testing.assert_array_equal(K.cpu().numpy(), r)
+        for (sp, d), st in zip(sps, sts):
+            E.line_params(sp, 1200.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=8.0, state=st)
+        del g
 
 
 def test_config0_fp64_and_fp32c_subsample(E):
