@@ -68,6 +68,10 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     log("bench: workload %s (%s)" % (args.workload, w.note))
+    if len(w.lines) > 1 or len(w.tp) > 1:
+        return bench_batched(args, w)
+    if args.workload == "c3":
+        return bench_adjoint(args, w)
     if args.copy_probe_mb > 0:
         n = args.copy_probe_mb * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
@@ -190,6 +194,155 @@ def main():
                      "model": "ops/eval A=%g B=%g C=%g (FP64 formulas, div/exp/trig x4)"
                               % (FLOPS_PER_EVAL["A"], FLOPS_PER_EVAL["B"], FLOPS_PER_EVAL["C"])},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args, w):
+    """Many species x many (T, P) in one graph-captured launch per pass
+    (autotuned tile size), FP32C; then the k-tables of every spectrum."""
+    import numpy as np
+    import torch
+
+    from heliosk import configs
+    from heliosk import engine as E
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    grid = E.make_grid(w.numin, w.numax, w.dnu)
+    species = []
+    for s_ in range(len(w.lines)):
+        arr = configs.species_arrays(w, s_)
+        species.append(E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"],
+                                 iso_mass=arr["iso_mass"]))
+    items, states = [], []
+    nspec = len(species) * len(w.tp)
+    Kall = torch.empty(nspec, grid.nx, dtype=torch.float64, device=dev)
+    evals = 0
+    k = 0
+    for sp in species:
+        for T, Pbar in w.tp:
+            st = E.line_params(sp, T, Pbar * configs.POBAR, Q=[1.0], cut_mode=0, cut=w.cut)
+            evals += E.pair_count(st, grid)[0]
+            states.append(st)
+            items.append((st, grid, 0.0, Kall[k]))
+            k += 1
+    torch.cuda.synchronize()
+    log("bench: setup %.1f s (%d spectra, %.4e evaluations)" % (time.time() - t0, nspec, evals))
+    # autotune on the largest species at the middle (T, P)
+    mid = (len(species) - 1) * len(w.tp) + len(w.tp) // 2
+    tile, tms = E.autotune(states[mid], grid, E.HK_VOIGT_FP32C)
+    log("bench: autotuned tile %d points (%s ms)" % (tile, {k_: round(v, 3) for k_, v in tms.items()}))
+    res = {}
+    for mode, name in ((E.HK_VOIGT_FP32C, "fp32c"), (E.HK_VOIGT_FP64, "fp64")):
+        if mode == E.HK_VOIGT_FP64 and args.fp64_steps <= 0:
+            continue
+        batch = E.VoigtBatch(items, mode=mode, tile_points=tile)
+        batch.launch()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            batch.launch()
+        steps = args.steps if mode == E.HK_VOIGT_FP32C else args.fp64_steps
+        for _ in range(args.warmup if mode == E.HK_VOIGT_FP32C else 0):
+            g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / steps
+        log("bench: %s batched Voigt %.3f ms per replay" % (name, res[name]))
+        if name == "fp32c":
+            K32 = Kall.clone()
+        del g, batch
+    rel = None
+    if "fp64" in res:
+        tau = 1e-12 * torch.amax(Kall, dim=1, keepdim=True)
+        rel = float(torch.max(torch.abs(K32 - Kall) / torch.maximum(torch.abs(Kall), tau)))
+    kt = E.KTable(grid, w.numax, w.nbins, w.nC, nspec=nspec)
+    kt(K32, kmin=0.0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    kt(K32, kmin=0.0)
+    e1.record()
+    torch.cuda.synchronize()
+    ktable_ms = e0.elapsed_time(e1)
+    t_step = (res["fp32c"] + ktable_ms) * 1e-3
+    out = {
+        "metric": "Voigt evals/s + k-table pts/s, batched FP32-comp vs plain FP64 at <=1e-5 rel err",
+        "value": evals / t_step, "unit": "Voigt evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (compensated) profile, f64 line params and k-fit",
+        "data": "synthetic (seeded generator, %d species, %d lines)" % (len(species), sum(w.lines)),
+        "config": {"workload": "%s: %s" % (w.name, w.note), "spectra": nspec,
+                   "grid_points": grid.nx, "cut_cm-1": w.cut, "bins": w.nbins, "nC": w.nC,
+                   "tile_points": tile, "parallelism": "single GPU"},
+        "voigt_evals_per_step": evals,
+        "voigt_batched_ms": res["fp32c"],
+        "voigt_evals_per_s_kernel": evals / (res["fp32c"] * 1e-3),
+        "fp64_voigt_batched_ms": res.get("fp64"),
+        "speedup_fp32c_vs_fp64": res["fp64"] / res["fp32c"] if "fp64" in res else None,
+        "fp32c_vs_fp64_max_rel_err": rel,
+        "precision_ok": (rel is not None and rel <= 1e-5),
+        "ktable_ms": ktable_ms,
+        "ktable_points_per_s": nspec * grid.nx / (ktable_ms * 1e-3),
+        "autotune_ms": tms,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args, w):
+    """Differentiation workload: FP64 forward + hand-written adjoint of a
+    binned log-mean loss, through the torch.autograd.Function."""
+    import numpy as np
+    import torch
+
+    from heliosk import configs
+    from heliosk import engine as E
+
+    dev = torch.device("cuda", 0)
+    arr = configs.species_arrays(w, 0)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    T, Pbar = w.tp[0]
+    P = Pbar * configs.POBAR
+    grid = E.make_grid(w.numin, w.numax, w.dnu)
+    wts = torch.from_numpy(np.random.default_rng(w.seed).uniform(0, 1, w.nbins)).to(dev)
+    st = E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut)
+    evals = E.pair_count(st, grid)[0]
+
+    def step():
+        Tt = torch.tensor(T, dtype=torch.float64, requires_grad=True)
+        Pt = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+        S = sp.S.clone().requires_grad_()
+        vy = sp.vy.clone().requires_grad_()
+        K = E.opacity(Tt, Pt, S, vy, sp, grid, Q=[1.0], cut_mode=0, cut=w.cut)
+        L = E.binned_log_mean(K, w.nbins, wts)
+        L.backward()
+        return L
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    out = {
+        "metric": "Voigt evals/s (forward + adjoint) of a binned-opacity loss",
+        "value": 2 * evals / t_step, "unit": "Voigt evaluations/s (forward and adjoint counted)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, %d lines)" % sp.nlines,
+        "config": {"workload": "%s: %s" % (w.name, w.note), "bins": w.nbins,
+                   "parallelism": "single GPU"},
+        "voigt_evals_per_step": evals, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

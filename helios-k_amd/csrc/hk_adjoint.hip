@@ -9,7 +9,8 @@
 //   dL/dvy_i  = S_i  sum_j gK_j dRe w/dy / sqrt(pi) * dy/dvy
 //   dL/dT, dL/dP = sum_i S_i sum_j gK_j [Re w dlnS + dRe w/dx dx + dRe w/dy dy]
 // with w'(z) = -2 z w + 2i/sqrt(pi):  dRe w/dx = -2 (x Re w - y Im w),
-// dRe w/dy = 2 (x Im w + y Re w) - 2/sqrt(pi).  The partition function is
+// dRe w/dy = 2 (x Im w + y Re w) - 2/sqrt(pi) for |z|^2 < 1e4, and from the
+// asymptotic series of w' beyond (the identity cancels to ~eps |z|^2).  The partition function is
 // held fixed.  Per-line T and P terms are summed afterwards by one
 // workgroup in a fixed order, so the result is deterministic.
 #include <math.h>
@@ -77,10 +78,15 @@ __global__ __launch_bounds__(256) void adjoint_kernel(
         const double g = gK[j];
         const double d = fma((double)j, dnu, numin) - nu;
         const double x = d * ia;
-        double re, im;
+        double re, im, dRe_dx, dRe_dy;
         w64(x, y, re, im);
-        const double dRe_dx = -2.0 * (x * re - y * im);
-        const double dRe_dy = 2.0 * (x * im + y * re) - 2.0 * kISqrtPi;
+        const double r2 = fma(y, y, x * x);
+        if (r2 >= kRM64) {
+            drew_far64(x, y, r2, dRe_dx, dRe_dy);
+        } else {
+            dRe_dx = -2.0 * (x * re - y * im);
+            dRe_dy = 2.0 * (x * im + y * re) - 2.0 * kISqrtPi;
+        }
         const double dx_dT = d * dia_dT;
         const double dx_dP = -delta[i] * ia + d * dia_dP;
         sre = fma(g, re, sre);

@@ -351,4 +351,21 @@ __device__ __forceinline__ void w64(double x, double y, double &re, double &im) 
     w916_64(x, y, re, im);
 }
 
+// dRe w/dx = Re w'(z) and dRe w/dy = -Im w'(z) for |z|^2 >= 1e4 from the
+// asymptotic series w'(z) = -(i/sqrt(pi)) u (1 + 3/2 u + 15/4 u^2 + 105/8 u^3),
+// u = 1/z^2 (next term below 6e-15 relative).  The identity
+// w' = -2zw + 2i/sqrt(pi) cancels to ~eps |z|^2 there.
+__device__ __forceinline__ void drew_far64(double x, double y, double r2, double &ddx,
+                                           double &ddy) {
+    const double inv = rcp64(r2);
+    const double ur = (x - y) * (x + y) * inv * inv, ui = -2.0 * (x * inv) * (y * inv);
+    double pr = 13.125 * ur + 3.75, pi = 13.125 * ui;
+    double tr = pr * ur - pi * ui, ti = pr * ui + pi * ur;
+    pr = tr + 1.5; pi = ti;
+    tr = pr * ur - pi * ui; ti = pr * ui + pi * ur;
+    pr = tr + 1.0; pi = ti;
+    ddx = (ur * pi + ui * pr) * kISqrtPi;
+    ddy = (ur * pr - ui * pi) * kISqrtPi;
+}
+
 }  // namespace hk

@@ -87,36 +87,6 @@ __global__ __launch_bounds__(kSortThreads) void bitonic_kernel(const double *__r
     for (int i = threadIdx.x; i < len; i += kSortThreads) dst[i] = s[i];
 }
 
-// One merge pass: runs of width w inside each segment are merged pairwise.
-__global__ __launch_bounds__(256) void merge_pass_kernel(const double *__restrict__ in,
-                                                         double *__restrict__ out,
-                                                         const Seg *__restrict__ segs,
-                                                         int64_t w) {
-    const Seg sg = segs[blockIdx.y];
-    const int64_t o = ((int64_t)blockIdx.x * 256 + threadIdx.x) * kMergeE;
-    if (o >= sg.len) return;
-    const int64_t p = o / (2 * w);
-    const int64_t ps = p * 2 * w;
-    const int64_t n0 = min(w, sg.len - ps);
-    const int64_t n1 = min(w, sg.len - ps - n0);
-    const double *A = in + sg.start + ps;
-    const double *B = A + n0;
-    const int64_t diag = o - ps;
-    int64_t lo = max((int64_t)0, diag - n1), hi = min(diag, n0);
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (A[mid] <= B[diag - mid - 1]) lo = mid + 1; else hi = mid;
-    }
-    int64_t i = lo, j = diag - lo;
-    double *dst = out + sg.start + o;
-    const int64_t cnt = min((int64_t)kMergeE, n0 + n1 - diag);
-    for (int64_t k = 0; k < cnt; ++k) {
-        const bool takeA = j >= n1 || (i < n0 && A[i] <= B[j]);
-        dst[k] = takeA ? A[i] : B[j];
-        if (takeA) ++i; else ++j;
-    }
-}
-
 // Merge pass with LDS staging: each workgroup emits kMB consecutive outputs
 // of one merged pair; the two input slices it needs are found by one merge
 // path search per workgroup, loaded coalesced into LDS, merged there (each

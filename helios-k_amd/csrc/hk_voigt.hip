@@ -1048,6 +1048,40 @@ extern "C" int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode, co
     return hk_voigt_batched_launch(&b, stream);
 }
 
+extern "C" int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, int mode,
+                                 double *K, void *ws, size_t ws_bytes, hk_tune *best, float *ms,
+                                 hk_stream_t stream) {
+    HK_CHECK_ARG(st && grid && K && ws && best, "hk_voigt_autotune: null argument");
+    constexpr int kCand[4] = {512, 1024, 2048, 4096};
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HK_HIP(hipEventCreate(&e0));
+    HK_HIP(hipEventCreate(&e1));
+    float bestms = 0.0f;
+    int bestw = 0, rc = HK_OK;
+    for (int c = 0; c < 4 && rc == HK_OK; ++c) {
+        hk_tune t = {kCand[c], 0};
+        rc = hk_voigt_accumulate(st, grid, 0.0, 0, mode, &t, K, ws, ws_bytes, stream);
+        if (rc != HK_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = HK_ERR_HIP; break; }
+        rc = hk_voigt_accumulate(st, grid, 0.0, 0, mode, &t, K, ws, ws_bytes, stream);
+        if (rc != HK_OK) break;
+        float dt = 0.0f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&dt, e0, e1) != hipSuccess) {
+            set_error("hk_voigt_autotune: event timing failed");
+            rc = HK_ERR_HIP;
+            break;
+        }
+        if (ms) ms[c] = dt;
+        if (bestw == 0 || dt < bestms) { bestms = dt; bestw = kCand[c]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == HK_OK) { best->tile_points = bestw; best->lines_per_stage = kStage; }
+    return rc;
+}
+
 extern "C" int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
                                    unsigned long long *counts, hk_stream_t stream) {
     HK_CHECK_ARG(st && grid && counts, "hk_voigt_pair_count: null argument");

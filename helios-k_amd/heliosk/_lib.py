@@ -89,6 +89,8 @@ def lib():
         L.hk_voigt_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, P(Tune),
                                           c_vp, c_vp, c_size, c_vp]
         L.hk_voigt_pair_count.argtypes = [P(LineState), P(Grid), c_vp, c_vp]
+        L.hk_voigt_autotune.argtypes = [P(LineState), P(Grid), c_int, c_vp, c_vp, c_size, P(Tune),
+                                        ctypes.POINTER(ctypes.c_float), c_vp]
         L.hk_voigt_batched_workspace_bytes.restype = c_size
         L.hk_voigt_batched_workspace_bytes.argtypes = [c_int, P(Spectrum), P(Tune)]
         L.hk_voigt_batched.argtypes = [c_int, P(Spectrum), c_int, P(Tune), c_vp, c_size, c_vp]
@@ -122,6 +124,7 @@ def check(rc):
 # Every symbol include/heliosk.h declares.
 EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state_init",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
+            "hk_voigt_autotune",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",
             "hk_voigt_batched_launch", "hk_ktable_workspace_bytes",
             "hk_ktable_prepare", "hk_ktable", "hk_ktable_reconstruct", "hk_ktable_plan_host",

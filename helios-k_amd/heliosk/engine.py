@@ -5,6 +5,7 @@ the HIP kernels of libheliosk.so.  The functions mirror the reference's
 stages: line parameters (S2_kernel), the Voigt sum (Line6A/B/C_Call), the
 k-table (heliosk.cu:2046-2193) and the adjoint.
 """
+import copy
 import ctypes
 
 import numpy as np
@@ -17,7 +18,7 @@ from ._lib import (HK_VOIGT_FP32C, HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineSta
 NA = 6.0221412927e23          # define.h:42
 
 __all__ = ["Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
-           "ktable",
+           "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
            "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
 
@@ -66,6 +67,20 @@ class Species:
         self.iso_sscale = list(iso_sscale)
         self.max_abs_delta = float(np.max(np.abs(delta))) if delta is not None and nl else 0.0
         self.nlines = nl
+
+    def with_fields(self, **fields):
+        """A shallow copy whose per-line device arrays (S, vy, ...) are
+        replaced by the given tensors (float64, on the device, sorted order)."""
+        c = copy.copy(self)
+        for k, v in fields.items():
+            if k not in ("nu", "S", "EL", "vy", "n", "A", "delta"):
+                raise KeyError(k)
+            v = v.detach().to(dtype=torch.float64).contiguous()
+            if v.numel() != self.nlines or v.device != self.nu.device:
+                raise ValueError("%s: %d values on %s, species has %d on %s"
+                                 % (k, v.numel(), v.device, self.nlines, self.nu.device))
+            setattr(c, k, v)
+        return c
 
     def lines(self):
         L = Lines()
@@ -152,6 +167,28 @@ def _spectra(items):
         arr[i].st = st.st; arr[i].grid = g; arr[i].kmin = float(kmin)
         arr[i].accumulate = 0; arr[i].K = K.data_ptr()
     return arr
+
+
+_TUNED = {}
+
+
+def autotune(state, grid, mode=HK_VOIGT_FP32C, stream=None, cache_key=None):
+    """Fastest main-pass tile size for this input (hk_voigt_autotune), cached
+    per cache_key when one is given.  Returns (tile_points, {size: ms})."""
+    if cache_key is not None and (cache_key, mode) in _TUNED:
+        return _TUNED[(cache_key, mode)]
+    K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
+    tune = _tune(0)
+    nbytes = lib().hk_voigt_workspace_bytes(ctypes.byref(grid), ctypes.byref(tune))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+    best = Tune(0, 0)
+    ms = (ctypes.c_float * 4)()
+    check(lib().hk_voigt_autotune(ctypes.byref(state.st), ctypes.byref(grid), int(mode), _ptr(K),
+                                  _ptr(ws), nbytes, ctypes.byref(best), ms, _stream(stream)))
+    out = (best.tile_points, dict(zip((512, 1024, 2048, 4096), list(ms))))
+    if cache_key is not None:
+        _TUNED[(cache_key, mode)] = out
+    return out
 
 
 def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
@@ -249,6 +286,46 @@ def adjoint(species, T, P, gK, grid, Q=None, cut_mode=0, cut=0.0, stream=None):
                                  ctypes.byref(grid), _ptr(gK.contiguous()), _ptr(gS), _ptr(gv),
                                  _ptr(gTP), _ptr(ws), nbytes, _stream(stream)))
     return gTP, gS, gv
+
+
+class VoigtOpacity(torch.autograd.Function):
+    """K(T, P, S, vy) on the grid (plain FP64 form) as a differentiable
+    function; backward runs the hand-written adjoint kernel, which
+    recomputes the profile instead of storing it.
+
+    forward(T, P, S, vy, species, grid, Q, cut_mode, cut, kmin): T [K] and
+    P [atm] are 0-dim float64 tensors, S (= S_file) and vy per line in the
+    species' sorted order.  The partition function Q is held fixed."""
+
+    @staticmethod
+    def forward(ctx, T, P, S, vy, species, grid, Q=None, cut_mode=0, cut=0.0, kmin=0.0):
+        sp = species.with_fields(S=S, vy=vy)
+        st = line_params(sp, float(T), float(P), Q=Q, cut_mode=cut_mode, cut=cut)
+        K = voigt(st, grid, kmin, HK_VOIGT_FP64)
+        ctx.save_for_backward(T, P, S, vy)
+        ctx.args = (species, grid, Q, cut_mode, cut)
+        return K
+
+    @staticmethod
+    def backward(ctx, gK):
+        T, P, S, vy = ctx.saved_tensors
+        species, grid, Q, cut_mode, cut = ctx.args
+        sp = species.with_fields(S=S, vy=vy)
+        gTP, gS, gv = adjoint(sp, float(T), float(P), gK.to(torch.float64).contiguous(), grid,
+                              Q=Q, cut_mode=cut_mode, cut=cut)
+        gT = gTP[0].to(T.device).reshape(T.shape)
+        gP = gTP[1].to(P.device).reshape(P.shape)
+        return gT, gP, gS, gv, None, None, None, None, None, None
+
+
+def opacity(T, P, S, vy, species, grid, Q=None, cut_mode=0, cut=0.0, kmin=0.0):
+    """Differentiable cross section, see VoigtOpacity."""
+    return VoigtOpacity.apply(T, P, S, vy, species, grid, Q, cut_mode, cut, kmin)
+
+
+def binned_log_mean(K, nbins, w):
+    """sum_b w_b log(mean of K over bin b) for nbins equal bins of the grid."""
+    return torch.sum(w * torch.log(K.view(nbins, -1).mean(dim=1)))
 
 
 def bin_segments(grid, numax, nbins):

@@ -144,6 +144,18 @@ int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
 int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
                         unsigned long long *counts, hk_stream_t stream);
 
+/* Autotuner (replaces the reference's launch self-tuning, heliosk.cu:1363-
+ * 1401): times hk_voigt_accumulate on this input for each tile size in
+ * {512, 1024, 2048, 4096} points (one untimed and one timed launch each,
+ * device events) and writes the fastest into *best; ms[4] (may be NULL)
+ * receives the timings.  K is overwritten; the workspace must be sized
+ * for the smallest tile (hk_voigt_workspace_bytes covers every size).
+ * Waits for the stream.  Results of different tile sizes agree to
+ * rounding, not bit for bit. */
+int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, int mode,
+                      double *K, void *workspace, size_t workspace_bytes,
+                      hk_tune *best, float *ms, hk_stream_t stream);
+
 /* Many spectra in one persistent launch (work queue over spectrum tiles). */
 typedef struct {
     hk_line_state st;

@@ -264,6 +264,27 @@ void hko_voigt_points(int64_t nl, const double *nu, const double *S,
 }
 
 /* ------------------------------------------------------------------ */
+/* dRe w/dx = Re w'(z) and dRe w/dy = -Im w'(z) for |z|^2 >= 1e4 from   */
+/* the asymptotic series w'(z) = -(i/sqrt(pi)) u sum_k (2k+1)!!/2^k u^k, */
+/* u = 1/z^2 (six terms: the next is below 1e-20 relative).  Forming    */
+/* them from w' = -2zw + 2i/sqrt(pi) instead cancels to ~eps |z|^2.     */
+/* ------------------------------------------------------------------ */
+static void drew_far(double x, double y, double r2, double *ddx, double *ddy)
+{
+    double inv = 1.0 / r2;
+    double ur = (x - y) * (x + y) * inv * inv, ui = -2.0 * (x * inv) * (y * inv);
+    static const double ck[6] = {1.0, 1.5, 3.75, 13.125, 59.0625, 324.84375};
+    double pr = ck[5], pi = 0.0;
+    for (int k = 4; k >= 0; --k) {
+        double tr = pr * ur - pi * ui, ti = pr * ui + pi * ur;
+        pr = tr + ck[k]; pi = ti;
+    }
+    double sr = ur * pr - ui * pi, si = ur * pi + ui * pr;
+    *ddx = si * HKO_ISQRTPI;
+    *ddy = sr * HKO_ISQRTPI;
+}
+
+/* ------------------------------------------------------------------ */
 /* Analytic gradient of L = sum_j gK_j K_j                             */
 /* with respect to T, P (gTP[0], gTP[1]) and per-line S0, vy0.         */
 /* The partition function Q is held fixed (constant ratio).            */
@@ -322,10 +343,15 @@ void hko_voigt_grad(int64_t nl,
             if (g == 0.0) continue;
             double d = fma((double)j, dnu, numin) - nu;
             double x = d * ia;
-            double re, im;
+            double re, im, dRe_dx, dRe_dy;
             hko_faddeeva(x, y, &re, &im);
-            double dRe_dx = -2.0 * (x * re - y * im);
-            double dRe_dy = 2.0 * (x * im + y * re) - 2.0 * HKO_ISQRTPI;
+            double r2 = x * x + y * y;
+            if (r2 >= 1e4) {
+                drew_far(x, y, r2, &dRe_dx, &dRe_dy);
+            } else {
+                dRe_dx = -2.0 * (x * re - y * im);
+                dRe_dy = 2.0 * (x * im + y * re) - 2.0 * HKO_ISQRTPI;
+            }
             double dx_dT = d * dia_dT;
             double dx_dP = -delta[i] * ia + d * dia_dP;
             sre += g * re;

@@ -116,8 +116,13 @@ def window(nu, hw, numin, dnu, Nx):
 
 def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
                  nthreads=0):
-    """K at grid points jidx (all points if None); lines sorted by nu."""
+    """K at grid points jidx (all points if None).  The C routine searches
+    the lines by centre, so a state whose (pressure-shifted) centres are not
+    ascending is put in order first."""
     cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
+    if cols[0].size > 1 and np.any(np.diff(cols[0]) < 0):
+        o = np.argsort(cols[0], kind="stable")
+        cols = [np.ascontiguousarray(c[o]) for c in cols]
     nl = cols[0].size
     if search_radius is None:
         hw = cols[4]

@@ -57,10 +57,14 @@ def test_adjoint_vs_central_differences(E):
     fdP = (loss(d, T, P * (1 + h)) - loss(d, T, P * (1 - h))) / (2 * h * P)
     assert abs(gTP[0] - fdT) <= 1e-4 * abs(fdT)
     assert abs(gTP[1] - fdP) <= 1e-4 * abs(fdP)
+    # per line: L is linear in K, so the line's own spectrum gives the
+    # difference of L without cancelling against the other lines
     for i in (0, 17, 59):
+        one = {k: (v[i:i + 1].copy() if isinstance(v, np.ndarray) and v.shape == d["nu"].shape
+                   else v) for k, v in d.items()}
         for key, g in (("S", gS), ("vy", gv)):
-            dp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}
-            dm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}
-            dp[key][i] *= 1 + h; dm[key][i] *= 1 - h
+            dp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in one.items()}
+            dm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in one.items()}
+            dp[key][0] *= 1 + h; dm[key][0] *= 1 - h
             fd = (loss(dp, T, P) - loss(dm, T, P)) / (2 * h * d[key][i])
-            assert abs(g[i] - fd) <= 1e-4 * abs(fd) + 1e-9 * np.max(np.abs(g)), (key, i, g[i], fd)
+            assert abs(g[i] - fd) <= 1e-4 * abs(fd), (key, i, g[i], fd)

@@ -228,6 +228,35 @@ def test_voigt_grad_vs_central_differences():
             assert abs(g[i] - fd) <= 1e-6 * abs(fd), (key, i, g[i], fd)
 
 
+def test_voigt_grad_far_field_lines():
+    """Lines whose profile on the grid lies at |z|^2 >> 1e4 (centres near
+    0 cm^-1: alpha_D -> 0, y and x of order 1e4-1e9).  The derivative there
+    comes from the asymptotic series of w'; a single line is differentiated
+    so the central difference is free of cancellation."""
+    L = _lines(1, seed=7)
+    numin, dnu, Nx = 0.0, 0.01, 5000
+    gK = np.random.default_rng(8).uniform(0.0, 1.0, Nx)
+    for nu, T, P in ((0.0314, 1200.0, 0.0987), (2.5, 300.0, 1.0), (0.5, 2500.0, 10.0)):
+        L["nu"] = np.array([nu])
+
+        def loss(LL, TT, PP):
+            st = O.line_params(LL, TT, PP, 0, 50.0)
+            return float(np.dot(gK, O.voigt_points(st, numin, dnu, Nx)))
+
+        gT, gP, gS, gv = O.voigt_grad(L, T, P, 0, 50.0, numin, dnu, Nx, gK)
+        st = O.line_params(L, T, P, 0, 50.0)
+        assert st["y"][0] > 100.0              # every point has |z|^2 > 1e4
+        h = 1e-5
+        for key, g in (("vy", gv[0]), ("S", gS[0])):
+            Lp = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in L.items()}
+            Lm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in L.items()}
+            Lp[key][0] *= 1 + h; Lm[key][0] *= 1 - h
+            fd = (loss(Lp, T, P) - loss(Lm, T, P)) / (2 * h * L[key][0])
+            assert abs(g - fd) <= 1e-7 * abs(fd), (nu, key, g, fd)
+        fdT = (loss(L, T * (1 + h), P) - loss(L, T * (1 - h), P)) / (2 * h * T)
+        assert abs(gT - fdT) <= 1e-6 * abs(fdT), (nu, gT, fdT)
+
+
 # ---------------------------------------------------------------- k-table
 def _cbin(path):
     return np.loadtxt(path)
