@@ -34,9 +34,9 @@ def q_at(T, Q):
     return (Q[j] - Q[j - 1]) * t + Q[j - 1]
 
 
-def make_species(tmp, name, cids, masses, abund, n, seed, numin, numax):
+def make_species(tmp, name, cids, masses, abund, n, seed, numin, numax, s_scale=1.0):
     rng = np.random.default_rng(seed)
-    lines = dict(nu=rng.uniform(numin, numax, n), S=10 ** rng.uniform(-25, -19, n),
+    lines = dict(nu=rng.uniform(numin, numax, n), S=s_scale * 10 ** rng.uniform(-25, -19, n),
                  EL=rng.uniform(0, 3000, n), A=rng.uniform(0, 2e7, n) * (rng.uniform(size=n) < 0.2),
                  delta=rng.uniform(-0.01, 0.01, n), gair=rng.uniform(0.01, 0.1, n),
                  gself=rng.uniform(0.1, 0.4, n), n=rng.uniform(0.3, 0.8, n))
@@ -120,16 +120,19 @@ def test_driver_text_outputs_vs_oracle(tmp_path):
 
 
 def test_driver_binary_mixture_pressures_and_kread(tmp_path):
+    # Units = 1 writes K * M/N_A in float32: line strengths per gram (S * N_A / M)
+    # keep that product in float32's normal range
     s1 = make_species(tmp_path, "01_a", ["11", "12"], [18.010565, 20.014811], [0.997317, 0.002],
-                      2000, 2, 995.0, 1015.0)
-    s2 = make_species(tmp_path, "02_b", ["21"], [43.98983], [0.984204], 1500, 3, 990.0, 1020.0)
+                      2000, 2, 995.0, 1015.0, s_scale=NA / 18.0)
+    s2 = make_species(tmp_path, "02_b", ["21"], [43.98983], [0.984204], 1500, 3, 990.0, 1020.0,
+                      s_scale=NA / 44.0)
     (tmp_path / "P.txt").write_text("0.7\n3.0\n")
     (tmp_path / "species.txt").write_text("01_a 0.6\n02_b 0.4\n")
     T, q, gF, cut = 987.0, 1.0, 1.0, 4.0
     F.write_param_dat(str(tmp_path / "param.dat"), name="m", T=T, P=1.0, PFile="P.txt",
                       SpeciesFile="species.txt", numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut,
                       doResampling=1, nC=NC, doStoreFullK=2, nbins=NBINS, qalphaL=q, gammaF=gF,
-                      Units=1, kmin=1e-48)
+                      Units=1, kmin=1e-25)
     run_driver(str(tmp_path))
     nx = int((NUMAX - NUMIN) / DNU + 0.5)
     meanMass = float(0.6 * np.sum(s1["abund"] * s1["masses"]) + 0.4 * np.sum(s2["abund"] * s2["masses"]))
@@ -140,10 +143,10 @@ def test_driver_binary_mixture_pressures_and_kread(tmp_path):
     assert cb["coeff"].shape == (2 * NBINS, NC)
     np.testing.assert_array_equal(cb["TP"], np.repeat([[T, 0.7], [T, 3.0]], NBINS, axis=0))
     # Units = 1: kmin is divided by the unit scale and printed so (heliosk.cu:697-704)
-    np.testing.assert_allclose(cb["kmin"], 1e-48 / unit, rtol=1e-15)
+    np.testing.assert_allclose(cb["kmin"], 1e-25 / unit, rtol=1e-15)
     for iP, P in enumerate((0.7, 3.0)):
         Ko = oracle_K([s1, s2], [0.6, 0.4], meanMass, T, P, q, gF, cut, NUMIN, DNU, nx,
-                      1e-48 / unit, units=1)
+                      1e-25 / unit, units=1)
         e = rel_err(Kf[iP].astype(np.float64), Ko * unit)
         print("P %g: float32 output vs oracle %.3e" % (P, e))
         assert e <= 1.2e-7
@@ -152,7 +155,7 @@ def test_driver_binary_mixture_pressures_and_kread(tmp_path):
     F.write_param_dat(str(tmp_path / "param.dat"), name="m", T=T, P=1.0, PFile="P.txt",
                       SpeciesFile="species.txt", numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut,
                       doResampling=1, nC=NC, doStoreFullK=-2, nbins=NBINS, qalphaL=q, gammaF=gF,
-                      Units=1, kmin=1e-48, pathToK="./")
+                      Units=1, kmin=1e-25, pathToK="./")
     run_driver(str(tmp_path))
     cb2 = F.read_cbin(str(tmp_path / "Out_m_cbin.dat"), NC)
     # the same k-tables up to the float32 rounding of the stored K
