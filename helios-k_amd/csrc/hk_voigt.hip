@@ -68,19 +68,26 @@ struct VArgs {
     double *K;
 };
 
-// LDS record of one line for one tile.  lo/hi: the line's cut window;
-// [r1lo, r1hi), [r2lo, r2hi), [r3lo, r3hi): conservative tile-relative index
-// intervals outside which x^2 + y^2 is >= R1, R2, R3 (nested; R3 bounds the
-// core).  A wave whose points miss an interval takes the branch-free path of
-// the next region out.
+// LDS record of one line for one tile.  lo/hi: the line's cut window
+// (tile-relative); code: 4 bits per wave, the evaluation path that wave takes
+// for this line (wave_code), decided once when the record is built.
 struct Rec64 {
     double nu, ia, y2, B, G, p2, p3, p4;
-    int32_t lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi;
+    int32_t lo, hi, code, pad;
 };
 
+// FP32 record, grouped by the path that reads it: x and r2; far wing
+// (v = inv (B + inv (1.5 B - 2 y^2 B inv)), inv = 1/r2); 4-node rule with
+// the strength folded into the weights and u_k = x^2 + (y^2 + t_k^2); 8-node
+// rule and the Doppler term; window and region intervals.
 struct Rec32 {
-    float r, vbh, vbl, y2, A, B, G, cy;
-    int32_t d0, lo, hi, r1lo, r1hi, r2lo, r2hi, r3lo, r3hi, dop, pad0, pad1;
+    float r, vbh, vbl, y2;
+    float B, B15, BC;
+    int32_t d0;
+    float Gw1, Gw2, ya1, ya2;
+    float G, A;
+    int32_t dop, code;
+    int32_t lo, hi, pad0, pad1;
 };
 
 // Line-core pass records: lo/hi bound the points of the tile that lie both
@@ -193,6 +200,36 @@ __device__ __forceinline__ bool misses(int32_t lo, int32_t hi, int wlo, int whi)
     return hi <= wlo || lo >= whi;
 }
 
+// Per-wave path code of one line: 0 = the window misses the wave's points;
+// otherwise 1 + 2 * path + (window covers the wave ? 0 : 1).  [r1lo, r1hi),
+// [r2lo, r2hi), [r3lo, r3hi) are conservative tile-relative intervals
+// outside which x^2 + y^2 >= R1 > R2 > R3; a wave that misses an interval
+// takes the branch-free path of the next region out.  The whole-wave
+// Gauss-Hermite paths form products of order (x^2+y^2)^3 (4 nodes, with the
+// strength folded in) and (x^2+y^2)^7 (8 nodes); r2max(w) bounds x^2 + y^2
+// over wave w, and a wave beyond mid_max / mid2_max takes the per-point path,
+// which uses those rules only below the far-wing threshold.
+enum { kPathFar = 0, kPathMid = 1, kPathMid2 = 2, kPathAny = 3 };
+
+template <int WPTS, class R2Max>
+__device__ __forceinline__ int32_t wave_codes(int32_t lo, int32_t hi, const int32_t (&ri)[6],
+                                              R2Max r2max, double mid_max, double mid2_max) {
+    int32_t code = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int wlo = w * WPTS, whi = wlo + WPTS;
+        if (misses(lo, hi, wlo, whi)) continue;
+        int path;
+        if (misses(ri[0], ri[1], wlo, whi)) path = kPathFar;
+        else if (misses(ri[2], ri[3], wlo, whi)) path = r2max(w) < mid_max ? kPathMid : kPathAny;
+        else if (misses(ri[4], ri[5], wlo, whi)) path = r2max(w) < mid2_max ? kPathMid2 : kPathAny;
+        else path = kPathAny;
+        const int full = (lo <= wlo && hi >= whi) ? 0 : 1;
+        code |= (1 + 2 * path + full) << (4 * w);
+    }
+    return code;
+}
+
 // FP64 per-line core constant q1 = sum_{n=1..13} e^{-a^2n^2} / (a^2 n^2 +
 // y^2), a = kA64, with e^{-a^2 n^2} and a^2 n^2 as literals.
 __device__ __forceinline__ double core_q1_64(double y) {
@@ -228,8 +265,6 @@ __device__ __forceinline__ float core_q1_32(float y) {
     return s;
 }
 
-// Paths of the per-(line, wave) dispatch.
-enum { kPathFar = 0, kPathMid = 1, kPathMid2 = 2, kPathAny = 3 };
 
 // ---------------------------------------------------------------------------
 // FP64 form.  Wave w owns the contiguous points [w*64*PPT, (w+1)*64*PPT) of
@@ -273,13 +308,19 @@ __device__ __forceinline__ void eval64(const Rec64 &qr, const double *xj, double
     }
 }
 
-template <bool FULL, int PPT>
-__device__ __forceinline__ void dispatch64(const Rec64 &q, const double *xj, double *acc, int jr0,
-                                           int wlo, int whi) {
-    if (misses(q.r1lo, q.r1hi, wlo, whi)) eval64<kPathFar, FULL, PPT>(q, xj, acc, jr0);
-    else if (misses(q.r2lo, q.r2hi, wlo, whi)) eval64<kPathMid, FULL, PPT>(q, xj, acc, jr0);
-    else if (misses(q.r3lo, q.r3hi, wlo, whi)) eval64<kPathMid2, FULL, PPT>(q, xj, acc, jr0);
-    else eval64<kPathAny, FULL, PPT>(q, xj, acc, jr0);
+template <int PPT>
+__device__ __forceinline__ void dispatch64(const Rec64 &q, int code, const double *xj, double *acc,
+                                           int jr0) {
+    switch (code) {
+        case 1: eval64<kPathFar, true, PPT>(q, xj, acc, jr0); break;
+        case 2: eval64<kPathFar, false, PPT>(q, xj, acc, jr0); break;
+        case 3: eval64<kPathMid, true, PPT>(q, xj, acc, jr0); break;
+        case 4: eval64<kPathMid, false, PPT>(q, xj, acc, jr0); break;
+        case 5: eval64<kPathMid2, true, PPT>(q, xj, acc, jr0); break;
+        case 6: eval64<kPathMid2, false, PPT>(q, xj, acc, jr0); break;
+        case 7: eval64<kPathAny, true, PPT>(q, xj, acc, jr0); break;
+        default: eval64<kPathAny, false, PPT>(q, xj, acc, jr0); break;
+    }
 }
 
 template <int PPT>
@@ -288,7 +329,7 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int wlo = wave * 64 * PPT, whi = wlo + 64 * PPT;
+    const int wlo = wave * 64 * PPT;
     const int jr0 = wlo + lane;
     const TileInfo ti = a.tiles[tile];
     const int64_t j0 = tile * W;
@@ -301,7 +342,7 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec64 r;
-        r.lo = 0; r.hi = 0;
+        r.lo = 0; r.hi = 0; r.code = 0;
         if (i < ti.hi) {
             const double nu = a.nu[i];
             int64_t wl, wh;
@@ -317,9 +358,11 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
                 const double jcd = rint((nu - a.numin) / a.dnu);
                 const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
                 const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
-                region_interval(kRF64, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
-                region_interval(kRM64, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
-                region_interval(kRB64, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
+                int32_t ri[6];
+                region_interval(kRF64, y2, rr, vb, d0, W, ri[0], ri[1]);
+                region_interval(kRM64, y2, rr, vb, d0, W, ri[2], ri[3]);
+                region_interval(kRB64, y2, rr, vb, d0, W, ri[4], ri[5]);
+                r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, [](int) { return 0.0; }, 1.0, 1.0);
             }
         }
         __syncthreads();
@@ -328,10 +371,9 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
             const Rec64 &q = rec[l];
-            const int32_t qlo = q.lo, qhi = q.hi;
-            if (qhi <= wlo || qlo >= whi) continue;          // window misses this wave
-            if (qlo <= wlo && qhi >= whi) dispatch64<true, PPT>(q, xj, acc, jr0, wlo, whi);
-            else dispatch64<false, PPT>(q, xj, acc, jr0, wlo, whi);
+            const int code = (__builtin_amdgcn_readfirstlane(q.code) >> (4 * wave)) & 15;
+            if (code == 0) continue;                         // window misses this wave
+            dispatch64<PPT>(q, code, xj, acc, jr0);
         }
     }
 #pragma unroll
@@ -353,29 +395,47 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
 }
 
 constexpr int kDT = 16;      // per-line core table stride (entries 1..kNT32 used)
+// Largest x^2 + y^2 over a wave for the whole-wave 4-node and 8-node FP32
+// paths: keeps G w_k u_k D_k (strengths scaled to <= 2^60) and D12 D34
+// below the FP32 range.
+constexpr double kMidMax32 = 1.0e5, kMid2Max32 = 3.0e4;
+
+// Strength-weighted 4-node Gauss-Hermite sum (the pair fold of gh4_sum32
+// with G folded into the weights).
+__device__ __forceinline__ float gh4_w32(const Rec32 &q, float x) {
+    constexpr float k1 = (float)(-4.0 * kGH4t1 * kGH4t1), k2 = (float)(-4.0 * kGH4t2 * kGH4t2);
+    const float u1 = fmaf(x, x, q.ya1), u2 = fmaf(x, x, q.ya2);
+    const float x2 = x * x;
+    const float D1 = fmaf(u1, u1, k1 * x2), D2 = fmaf(u2, u2, k2 * x2);
+    const float num = fmaf(q.Gw1 * u1, D2, (q.Gw2 * u2) * D1);
+    return num * rcp32(D1 * D2);
+}
+
+__device__ __forceinline__ float far32(const Rec32 &q, float r2) {
+    const float inv = rcp32(r2);
+    return inv * fmaf(fmaf(q.BC, inv, q.B15), inv, q.B);
+}
 
 template <int PATH, bool FULL, int PPT>
 __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
-    const float r0 = qr.r, vbh = qr.vbh, vbl = qr.vbl, y2 = qr.y2, B = qr.B, G = qr.G, cy = qr.cy;
+    const float r0 = qr.r, vbh = qr.vbh, vbl = qr.vbl, y2 = qr.y2, G = qr.G;
     const int32_t lo = qr.lo, hi = qr.hi;
     const float m0 = (float)(jr0 + qr.d0);
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const float m = m0 + (float)(64 * k);
         const float x = fmaf(m, vbh, fmaf(m, vbl, r0));
-        const float r2 = fmaf(x, x, y2);
         float v;
         if (PATH == kPathFar) {
-            const float inv = rcp32(r2);
-            v = (B * inv) * fmaf(fmaf(cy, inv, 1.5f), inv, 1.0f);
+            v = far32(qr, fmaf(x, x, y2));
         } else if (PATH == kPathMid) {
-            v = G * gh4_sum32(x * x, r2);
+            v = gh4_w32(qr, x);
         } else {
+            const float r2 = fmaf(x, x, y2);
             if (r2 >= kRF32) {
-                const float inv = rcp32(r2);
-                v = (B * inv) * fmaf(fmaf(cy, inv, 1.5f), inv, 1.0f);
+                v = far32(qr, r2);
             } else if (r2 >= kRM32) {
-                v = G * gh4_sum32(x * x, r2);
+                v = gh4_w32(qr, x);
             } else if (PATH == kPathMid2 || r2 >= kRB32) {
                 v = G * gh8_sum32(x * x, r2);
                 if (qr.dop) v = fmaf(qr.A, __expf(-x * x), v);
@@ -391,21 +451,17 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
     }
 }
 
-template <bool FULL, int PPT>
-__device__ __forceinline__ void dispatch32(const Rec32 &q, float *part, int jr0, int wlo, int whi) {
-    if (misses(q.r1lo, q.r1hi, wlo, whi)) {
-        eval32<kPathFar, FULL, PPT>(q, part, jr0);
-    } else if (misses(q.r2lo, q.r2hi, wlo, whi)) {
-        // 4-node rule for the whole wave unless x is so large that its
-        // single-reciprocal form could overflow FP32
-        const float m0 = (float)(wlo + q.d0), m1 = (float)(whi - 1 + q.d0);
-        const float x0 = fmaf(m0, q.vbh, q.r), x1 = fmaf(m1, q.vbh, q.r);
-        if (fmaxf(x0 * x0, x1 * x1) < 1.0e9f) eval32<kPathMid, FULL, PPT>(q, part, jr0);
-        else eval32<kPathMid2, FULL, PPT>(q, part, jr0);
-    } else if (misses(q.r3lo, q.r3hi, wlo, whi)) {
-        eval32<kPathMid2, FULL, PPT>(q, part, jr0);
-    } else {
-        eval32<kPathAny, FULL, PPT>(q, part, jr0);
+template <int PPT>
+__device__ __forceinline__ void dispatch32(const Rec32 &q, int code, float *part, int jr0) {
+    switch (code) {
+        case 1: eval32<kPathFar, true, PPT>(q, part, jr0); break;
+        case 2: eval32<kPathFar, false, PPT>(q, part, jr0); break;
+        case 3: eval32<kPathMid, true, PPT>(q, part, jr0); break;
+        case 4: eval32<kPathMid, false, PPT>(q, part, jr0); break;
+        case 5: eval32<kPathMid2, true, PPT>(q, part, jr0); break;
+        case 6: eval32<kPathMid2, false, PPT>(q, part, jr0); break;
+        case 7: eval32<kPathAny, true, PPT>(q, part, jr0); break;
+        default: eval32<kPathAny, false, PPT>(q, part, jr0); break;
     }
 }
 
@@ -415,7 +471,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int wlo = wave * 64 * PPT, whi = wlo + 64 * PPT;
+    const int wlo = wave * 64 * PPT;
     const int jr0 = wlo + lane;
     const TileInfo ti = a.tiles[tile];
     const int64_t j0 = tile * W;
@@ -426,7 +482,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec32 r;
-        r.lo = 0; r.hi = 0;
+        r.lo = 0; r.hi = 0; r.code = 0;
         if (i < ti.hi) {
             const double nu = a.nu[i];
             int64_t wl, wh;
@@ -446,17 +502,34 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.vbl = (float)(vb - (double)vbh);
                 r.y2 = yf * yf;
                 r.A = (float)A;
-                r.B = (float)(A * y * kISqrtPi);
-                r.G = (float)(A * y * (2.0 / kPi));
-                r.cy = -2.0f * yf * yf;
+                const double B = A * y * kISqrtPi, G = A * y * (2.0 / kPi);
+                const double yy = (double)r.y2;
+                r.B = (float)B;
+                r.B15 = (float)(1.5 * B);
+                // -2 y^2 B may exceed FP32 only for y > ~1e6, where the term
+                // it multiplies (-2 y^2 / r2^2 <= 2 / r2) is negligible
+                r.BC = (float)fmax(-2.0 * yy * B, -1.0e37);
+                r.G = (float)G;
+                r.Gw1 = (float)(G * kGH4w1);
+                r.Gw2 = (float)(G * kGH4w2);
+                r.ya1 = (float)(yy + kGH4t1 * kGH4t1);
+                r.ya2 = (float)(yy + kGH4t2 * kGH4t2);
                 r.d0 = (int32_t)d0;
                 r.lo = (int32_t)(clo - j0);
                 r.hi = (int32_t)(chi - j0);
                 r.dop = yf < kY0 ? 1 : 0;
                 const double y2 = (double)yf * (double)yf;
-                region_interval((double)kRF32, y2, rr, vb, d0, W, r.r1lo, r.r1hi);
-                region_interval((double)kRM32, y2, rr, vb, d0, W, r.r2lo, r.r2hi);
-                region_interval((double)kRB32, y2, rr, vb, d0, W, r.r3lo, r.r3hi);
+                int32_t ri[6];
+                region_interval((double)kRF32, y2, rr, vb, d0, W, ri[0], ri[1]);
+                region_interval((double)kRM32, y2, rr, vb, d0, W, ri[2], ri[3]);
+                region_interval((double)kRB32, y2, rr, vb, d0, W, ri[4], ri[5]);
+                // x is monotonic over a wave: its largest x^2 is at an end
+                auto r2max = [&](int w) {
+                    const double xa = ((double)(w * 64 * PPT) + d0) * vb + rr;
+                    const double xb = ((double)((w + 1) * 64 * PPT - 1) + d0) * vb + rr;
+                    return fmax(xa * xa, xb * xb) + y2;
+                };
+                r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, r2max, kMidMax32, kMid2Max32);
             }
         }
         // the previous stage's readers of rec[] are done after this
@@ -466,12 +539,9 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
             const Rec32 &q = rec[l];
-            const int32_t qlo = q.lo, qhi = q.hi;
-            if (qlo >= qhi) continue;
-            if (!(qhi <= wlo || qlo >= whi)) {
-                if (qlo <= wlo && qhi >= whi) dispatch32<true, PPT>(q, part, jr0, wlo, whi);
-                else dispatch32<false, PPT>(q, part, jr0, wlo, whi);
-            }
+            const int code = (__builtin_amdgcn_readfirstlane(q.code) >> (4 * wave)) & 15;
+            if (code == 0) continue;                         // window misses this wave
+            dispatch32<PPT>(q, code, part, jr0);
             if (++since == 32) {
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }

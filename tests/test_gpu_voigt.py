@@ -73,6 +73,9 @@ VOIGT_CASES = [
     ("dense-cores", 2000.0, 9.87, 0, 5.0, 20000, (10000.0, 10010.0, 0.001), (9995.0, 10015.0)),
     # each line core spans thousands of points (many line-core tiles)
     ("wide-cores", 3000.0, 1e-4, 0, 1.0, 100, (1000.0, 1004.0, 0.00001), (999.5, 1004.5)),
+    # Doppler widths far below the grid step (x steps of ~100 per point):
+    # whole-wave Gauss-Hermite paths would span x^2 up to 1e10
+    ("narrow-lines", 100.0, 1e-6, 0, 100.0, 200, (0.0, 1000.0, 0.01), (0.0, 1000.0)),
 ]
 
 
@@ -89,6 +92,20 @@ def test_voigt_vs_oracle(E, case):
     # exactly-empty points agree (shared FP64 integer windows)
     np.testing.assert_array_equal(K64 == 0.0, Ko == 0.0)
     np.testing.assert_array_equal(K32 == 0.0, Ko == 0.0)
+
+
+def test_voigt_centres_near_zero(E):
+    """Centres down to 1e-4 cm^-1: alpha_D -> 0, so y reaches ~1e8 and x
+    ~1e9; every FP32 intermediate must stay finite."""
+    rng = np.random.default_rng(31)
+    d = random_lines(300, seed=31, numin=0.0, numax=12.0)
+    d["nu"] = np.sort(10 ** rng.uniform(-4, np.log10(12.0), 300))
+    grid = _grid(E, 0.0, 10.0, 0.001)
+    Ko, K64, K32 = _compare(E, d, 1500.0, 1.0, grid, 0, 5.0)
+    e64, e32 = rel_err(K64, Ko), rel_err(K32, Ko)
+    print("near-zero centres: FP64 %.3e  FP32C %.3e" % (e64, e32))
+    assert np.all(np.isfinite(K32)) and np.all(np.isfinite(K64))
+    assert e64 <= FP64_TOL and e32 <= FP32C_TOL
 
 
 def test_voigt_edges_and_kmin(E):
