@@ -76,18 +76,18 @@ struct Rec64 {
     int32_t lo, hi, code, pad;
 };
 
-// FP32 record, grouped by the path that reads it: x and r2; far wing
-// (v = inv (B + inv (1.5 B - 2 y^2 B inv)), inv = 1/r2); 4-node rule with
-// the strength folded into the weights and u_k = x^2 + (y^2 + t_k^2); 8-node
-// rule and the Doppler term; window and region intervals.
+// FP32 record, grouped by the path that reads it: x and r2; the asymptotic
+// series of the wings with the strength folded into the coefficients
+// (far: B, B15, BC; mid: q2..q6, see far4_32); the 8-node rule and the
+// Doppler term; window and dispatch word.
 struct Rec32 {
     float r, vbh, vbl, y2;
     float B, B15, BC;
     int32_t d0;
-    float Gw1, Gw2, ya1, ya2;
-    float G, A;
-    int32_t dop, code;
-    int32_t lo, hi, pad0, pad1;
+    float q2, q3, q4, q5;
+    float q6, G, A;
+    int32_t dop;
+    int32_t code, lo, hi, pad;
 };
 
 // Line-core pass records: lo/hi bound the points of the tile that lie both
@@ -395,25 +395,34 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
 }
 
 constexpr int kDT = 16;      // per-line core table stride (entries 1..kNT32 used)
-// Largest x^2 + y^2 over a wave for the whole-wave 4-node and 8-node FP32
-// paths: keeps G w_k u_k D_k (strengths scaled to <= 2^60) and D12 D34
-// below the FP32 range.
-constexpr double kMidMax32 = 1.0e5, kMid2Max32 = 3.0e4;
+// Largest x^2 + y^2 over a wave for the whole-wave 8-node FP32 path: keeps
+// D12 D34 ~ r2^8 below the FP32 range.  The whole-wave series path has no
+// such limit (its coefficients are bounded by y^2 < 4096).
+constexpr double kMidMax32 = 1.0e300, kMid2Max32 = 3.0e4;
 
-// Strength-weighted 4-node Gauss-Hermite sum (the pair fold of gh4_sum32
-// with G folded into the weights).
-__device__ __forceinline__ float gh4_w32(const Rec32 &q, float x) {
-    constexpr float k1 = (float)(-4.0 * kGH4t1 * kGH4t1), k2 = (float)(-4.0 * kGH4t2 * kGH4t2);
-    const float u1 = fmaf(x, x, q.ya1), u2 = fmaf(x, x, q.ya2);
-    const float x2 = x * x;
-    const float D1 = fmaf(u1, u1, k1 * x2), D2 = fmaf(u2, u2, k2 * x2);
-    const float num = fmaf(q.Gw1 * u1, D2, (q.Gw2 * u2) * D1);
-    return num * rcp32(D1 * D2);
-}
-
+// Wings from the asymptotic series w(z) ~ (i/sqrt(pi)) sum_n c_n z^-(2n+1),
+// c_n = (2n-1)!!/2^n.  With inv = 1/r2 and t = y^2 inv its real part is
+//   Re w sqrt(pi) = y inv [1 + inv (3/2 - 2t) + inv^2 (15/4 - 15t + 12t^2)
+//                   + inv^3 (105/8 - 105t + 210t^2 - 120t^3) + ...],
+// regrouped in powers of inv with per-line coefficients in y^2 and the
+// strength B = A y / sqrt(pi) folded in.
+//  far32 (r2 >= 4096): terms n <= 1, truncation ~0.75/r2^2 (< 5e-8);
+//  far4_32 (100 <= r2 < 4096, y^2 < 4096): terms n <= 3, truncation
+//  ~59/r2^4 relative (< 1e-6 at r2 = 100).
 __device__ __forceinline__ float far32(const Rec32 &q, float r2) {
     const float inv = rcp32(r2);
     return inv * fmaf(fmaf(q.BC, inv, q.B15), inv, q.B);
+}
+
+__device__ __forceinline__ float far4_32(const Rec32 &q, float r2) {
+    const float inv = rcp32(r2);
+    float p = fmaf(q.q6, inv, q.q5);
+    p = fmaf(p, inv, q.q4);
+    p = fmaf(p, inv, q.q3);
+    p = fmaf(p, inv, q.q2);
+    p = fmaf(p, inv, q.B15);
+    p = fmaf(p, inv, q.B);
+    return inv * p;
 }
 
 template <int PATH, bool FULL, int PPT>
@@ -429,13 +438,13 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
         if (PATH == kPathFar) {
             v = far32(qr, fmaf(x, x, y2));
         } else if (PATH == kPathMid) {
-            v = gh4_w32(qr, x);
+            v = far4_32(qr, fmaf(x, x, y2));
         } else {
             const float r2 = fmaf(x, x, y2);
             if (r2 >= kRF32) {
                 v = far32(qr, r2);
             } else if (r2 >= kRM32) {
-                v = gh4_w32(qr, x);
+                v = far4_32(qr, r2);
             } else if (PATH == kPathMid2 || r2 >= kRB32) {
                 v = G * gh8_sum32(x * x, r2);
                 if (qr.dop) v = fmaf(qr.A, __expf(-x * x), v);
@@ -510,10 +519,13 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 // it multiplies (-2 y^2 / r2^2 <= 2 / r2) is negligible
                 r.BC = (float)fmax(-2.0 * yy * B, -1.0e37);
                 r.G = (float)G;
-                r.Gw1 = (float)(G * kGH4w1);
-                r.Gw2 = (float)(G * kGH4w2);
-                r.ya1 = (float)(yy + kGH4t1 * kGH4t1);
-                r.ya2 = (float)(yy + kGH4t2 * kGH4t2);
+                // series coefficients of far4_32, used only where y^2 < 4096
+                const double Bm = yy < (double)kRF32 ? B : 0.0;
+                r.q2 = (float)((3.75 - 2.0 * yy) * Bm);
+                r.q3 = (float)((13.125 - 15.0 * yy) * Bm);
+                r.q4 = (float)((12.0 * yy - 105.0) * yy * Bm);
+                r.q5 = (float)(210.0 * yy * yy * Bm);
+                r.q6 = (float)(-120.0 * yy * yy * yy * Bm);
                 r.d0 = (int32_t)d0;
                 r.lo = (int32_t)(clo - j0);
                 r.hi = (int32_t)(chi - j0);
