@@ -433,13 +433,14 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const float m = m0 + (float)(64 * k);
-        const float x = fmaf(m, vbh, fmaf(m, vbl, r0));
         float v;
-        if (PATH == kPathFar) {
-            v = far32(qr, fmaf(x, x, y2));
-        } else if (PATH == kPathMid) {
-            v = far4_32(qr, fmaf(x, x, y2));
+        if (PATH == kPathFar || PATH == kPathMid) {
+            // r2 >= 100 on these paths: |d ln v / dx| <= 2|x|/r2, so the
+            // relative error 6e-8 |x| of x = m fl(vb) + r moves v by < 1.2e-7
+            const float x = fmaf(m, vbh, r0);
+            v = PATH == kPathFar ? far32(qr, fmaf(x, x, y2)) : far4_32(qr, fmaf(x, x, y2));
         } else {
+            const float x = fmaf(m, vbh, fmaf(m, vbl, r0));
             const float r2 = fmaf(x, x, y2);
             if (r2 >= kRF32) {
                 v = far32(qr, r2);
@@ -738,27 +739,35 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
     if (touched && j < a.nx) a.K[j] += ldexp((double)hi + (double)lo, ti.e);
 }
 
+// Tile b is workgroup b, so the round-robin dealing of workgroups to the 8
+// XCDs interleaves the grid over them.  The work per tile grows with the
+// wavenumber (Doppler widths scale with nu), and this interleaving keeps the
+// XCDs balanced; giving each XCD a contiguous range instead (better L2 reuse
+// of line data) measured 1.1x slower for the main pass and 2x for the
+// line-core pass.
 template <int MODE, int PPT>
 __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
+    const int64_t tile = blockIdx.x;
     if (MODE == HK_VOIGT_FP64) {
         __shared__ Rec64 rec[kStage];
-        tile_f64<PPT>(a, blockIdx.x, rec);
+        tile_f64<PPT>(a, tile, rec);
     } else {
         __shared__ Rec32 rec[kStage];
-        tile_f32c<PPT>(a, blockIdx.x, rec);
+        tile_f32c<PPT>(a, tile, rec);
     }
 }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
     __shared__ int cnt[kThreads / 64];
+    const int64_t tile = blockIdx.x;
     if (MODE == HK_VOIGT_FP64) {
         __shared__ Core64 rec[kStage];
-        core_f64(a, blockIdx.x, rec, cnt);
+        core_f64(a, tile, rec, cnt);
     } else {
         __shared__ Core32 rec[kStage];
         __shared__ float dtab[kStage * kDT];
-        core_f32c(a, blockIdx.x, rec, dtab, cnt);
+        core_f32c(a, tile, rec, dtab, cnt);
     }
 }
 
