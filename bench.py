@@ -33,10 +33,36 @@ FLOPS_PER_EVAL = {"A": 10.0, "B": 22.0, "C": 400.0}
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
 PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
 PEAK_HBM_GBS = 8000.0
+# The two launches of one FP32C Voigt sum at the default 2048-point tiles.
+VOIGT_KERNELS = ("void hk::voigt_tiles_kernel<1, 8>(hk::VArgs)",
+                 "void hk::voigt_core_kernel<1>(hk::VArgs)")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def hbm_traffic(kernels, workload):
+    """HBM bytes of one launch of each of `kernels`, summed, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs of this
+    benchmark, each scaled by the factor the same counter reads for a 1 GiB
+    reduction / fill), or None when no summary for this workload exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_counters_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    a = text.find("```json")
+    if a < 0:
+        return None
+    b = text.find("```", a + 7)
+    d = json.loads(text[a + 7:b])
+    total = 0.0
+    for name in kernels:
+        k = d["kernels"].get(name)
+        if k is None:
+            return None
+        total += (k["fetch_kib"] * d["fetch_factor"] + k["write_kib"] * d["write_factor"]) * 1024.0
+    return total
 
 
 def main():
@@ -190,7 +216,10 @@ def main():
         "ktable_points_per_s": grid.nx / (ktable_ms * 1e-3),
         "ktable_hbm_GBs_algorithmic": 8.0 * grid.nx / (ktable_ms * 1e-3) / 1e9,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": hbm_traffic(VOIGT_KERNELS, args.workload),
+                     "kernel": "one Voigt sum = main pass + line-core pass (%s)"
+                               % " + ".join(VOIGT_KERNELS),
                      "model": "ops/eval A=%g B=%g C=%g (FP64 formulas, div/exp/trig x4)"
                               % (FLOPS_PER_EVAL["A"], FLOPS_PER_EVAL["B"], FLOPS_PER_EVAL["C"])},
         "cpu_baseline": cpu,
