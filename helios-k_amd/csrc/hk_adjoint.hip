@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(
     const double *__restrict__ nu0, const double *__restrict__ S0,
     const double *__restrict__ EL, const double *__restrict__ A,
     const double *__restrict__ delta, const double *__restrict__ vy0,
-    const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTabA tab,
+    const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTabA tab, int form,
     int64_t nl, double T, double P, int cut_mode, double cut, double numin, double dnu,
     int64_t nx, const double *__restrict__ gK, double *__restrict__ gS,
     double *__restrict__ gvy, double *__restrict__ gTl, double *__restrict__ gPl) {
@@ -44,33 +44,42 @@ __global__ __launch_bounds__(256) void adjoint_kernel(
     const double c = kH * kC / kKB;
     const int k = iso ? min(max(iso[i], 0), HK_MAX_ISO - 1) : 0;
     const double ialphaD_h = kC * sqrt(tab.mass[k] / (2.0 * kKB * T));
-    const double F0 = 1.0 / tab.Q[k] * tab.ab[k] * tab.ss[k];
-    const double nu = fma(delta[i], P, nu0[i]);
+    const bool hitran = form == HK_FORM_HITRAN;
+    const double F0 = form == HK_FORM_SUPERLINE ? tab.ab[k] * tab.ss[k]
+                                                : 1.0 / tab.Q[k] * tab.ab[k] * tab.ss[k];
+    const double dlt = hitran ? delta[i] : 0.0;
+    const double nu = fma(dlt, P, nu0[i]);
     if (nu == 0.0) {
         if (lane == 0) { gS[i] = 0.0; gvy[i] = 0.0; gTl[i] = 0.0; gPl[i] = 0.0; }
         return;
     }
     const double ia = ialphaD_h / nu;
     const double E = EL[i];
-    const double B = exp(-E * c / T + E * c / kT0);
+    // S = S0 F with F = F0 B R ia; B, R per form (HK_FORM_*): HITRAN relative
+    // to T0, ExoMol-family absolute, super-lines 1.
     const double eT = exp(-c * nu / T), e0 = exp(-c * nu / kT0);
-    const double Rr = (1.0 - eT) / (1.0 - e0);
+    const double B = hitran ? exp(-E * c / T + E * c / kT0)
+                            : (form == HK_FORM_EXOMOL ? exp(-c / T * E) : 1.0);
+    const double Rr = hitran ? (1.0 - eT) / (1.0 - e0)
+                             : (form == HK_FORM_EXOMOL ? 1.0 - eT : 1.0);
     const double F = F0 * B * Rr * ia;
     const double S = S0[i] * F;
     const double pw = pow(kT0 / T, nexp[i]);
-    const double aLp = vy0[i] * P * pw;
-    const double aL = aLp + A[i] / (4.0 * kPi * kC);
+    const double Pscale = hitran ? 1.0 / kPOatm : 1.0 / kPObar;
+    const double aLp = vy0[i] * P * Pscale * pw;
+    const double aL = aLp + (form == HK_FORM_SUPERLINE ? 0.0 : A[i] / (4.0 * kPi * kC));
     const double y = aL * ia;
     const double hw = cut_mode == 1 ? cut * aL : (cut_mode == 2 ? cut / ia : cut);
-    const double dlnR_dT = -eT * (c * nu / (T * T)) / (1.0 - eT);
+    const bool super = form == HK_FORM_SUPERLINE;
+    const double dlnR_dT = super ? 0.0 : -eT * (c * nu / (T * T)) / (1.0 - eT);
     const double dlnR_dnu = eT * (c / T) / (1.0 - eT) - e0 * (c / kT0) / (1.0 - e0);
-    const double dlnS_dT = c * E / (T * T) + dlnR_dT - 0.5 / T;
-    const double dlnS_dP = delta[i] * (dlnR_dnu - 1.0 / nu);
-    const double dia_dT = -0.5 * ia / T, dia_dP = -ia * delta[i] / nu;
-    const double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * pw;
+    const double dlnS_dT = (super ? 0.0 : c * E / (T * T)) + dlnR_dT - 0.5 / T;
+    const double dlnS_dP = dlt * (dlnR_dnu - 1.0 / nu);
+    const double dia_dT = -0.5 * ia / T, dia_dP = -ia * dlt / nu;
+    const double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * Pscale * pw;
     const double dy_dT = daL_dT * ia + aL * dia_dT;
     const double dy_dP = daL_dP * ia + aL * dia_dP;
-    const double dy_dvy = P * pw * ia;
+    const double dy_dvy = P * Pscale * pw * ia;
     int64_t jlo, jhi;
     cut_window(nu, hw, numin, dnu, nx, jlo, jhi);
     double sre = 0.0, sdy = 0.0, sT = 0.0, sP = 0.0;
@@ -88,7 +97,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(
             dRe_dy = 2.0 * (x * im + y * re) - 2.0 * kISqrtPi;
         }
         const double dx_dT = d * dia_dT;
-        const double dx_dP = -delta[i] * ia + d * dia_dP;
+        const double dx_dP = -dlt * ia + d * dia_dP;
         sre = fma(g, re, sre);
         sdy = fma(g, dRe_dy, sdy);
         sT = fma(g, re * dlnS_dT + dRe_dx * dx_dT + dRe_dy * dy_dT, sT);
@@ -131,6 +140,8 @@ extern "C" int hk_voigt_adjoint(const hk_lines *L, const hk_tp *tp, const hk_cut
     HK_CHECK_ARG(L && tp && cut && g && gK && gS && gvy && gTP && ws,
                  "hk_voigt_adjoint: null argument");
     HK_CHECK_ARG(L->niso >= 1 && L->niso <= HK_MAX_ISO, "hk_voigt_adjoint: niso %d", L->niso);
+    HK_CHECK_ARG(L->form >= HK_FORM_HITRAN && L->form <= HK_FORM_SUPERLINE,
+                 "hk_voigt_adjoint: form %d", L->form);
     HK_CHECK_ARG(g->nx > 0 && g->dnu > 0.0, "hk_voigt_adjoint: bad grid");
     if (ws_bytes < hk_voigt_adjoint_workspace_bytes(L->nlines)) {
         hk::set_error("hk_voigt_adjoint: workspace %zu bytes, %zu needed", ws_bytes,
@@ -151,7 +162,8 @@ extern "C" int hk_voigt_adjoint(const hk_lines *L, const hk_tp *tp, const hk_cut
     hipStream_t s = (hipStream_t)stream;
     if (L->nlines > 0) {
         hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)((L->nlines + 3) / 4)), dim3(256), 0, s,
-                           L->nu, L->S, L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, L->nlines,
+                           L->nu, L->S, L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, L->form,
+                           L->nlines,
                            tp->T, tp->P, cut->cut_mode, cutv, g->numin, g->dnu, g->nx, gK, gS, gvy,
                            gTl, gPl);
         HK_LAUNCH_CHECK();

@@ -10,6 +10,7 @@ namespace hk {
 // Physical constants of define.h:36-43.
 constexpr double kT0 = 296.0;
 constexpr double kPOatm = 1.0;
+constexpr double kPObar = 0.986923;          // define.h:37
 constexpr double kKB = 1.3806489e-16;
 constexpr double kH = 6.62606957e-27;
 constexpr double kC = 2.99792458e10;

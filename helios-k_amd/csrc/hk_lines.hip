@@ -2,7 +2,8 @@
 //
 // One thread per line computes, in FP64 and in the reference's order of
 // operations, the per-line part of readFile (host.h:641-644) followed by
-// S2_kernel (voigt.h:338-356):
+// S2_kernel (voigt.h:338-356) -- or, for ExoMol-family and super-line lists,
+// Sf_kernel / SfSuper_kernel (voigt.h:365-420; see HK_FORM_* in heliosk.h):
 //   1/alpha_D = c sqrt(m / (2 kB T)) / nu,    nu = nu0 + delta P
 //   S = S_file/Q*Ab*Sscale * exp(-c2 E/T + c2 E/T0)
 //       * (1 - exp(-c2 nu/T)) / (1 - exp(-c2 nu/T0)) * (1/alpha_D)
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(256) void line_params_kernel(
     const double *__restrict__ nu0, const double *__restrict__ S0,
     const double *__restrict__ EL, const double *__restrict__ A,
     const double *__restrict__ delta, const double *__restrict__ vy0,
-    const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTab tab,
+    const double *__restrict__ nexp, const int32_t *__restrict__ iso, IsoTab tab, int form,
     int64_t n, double T, double P, int cut_mode, double cut,
     double *__restrict__ o_nu, double *__restrict__ o_S, double *__restrict__ o_ia,
     double *__restrict__ o_y, double *__restrict__ o_hw, double *__restrict__ o_ey,
@@ -41,14 +42,23 @@ __global__ __launch_bounds__(256) void line_params_kernel(
         const double c = kH * kC / kKB;
         int k = iso ? min(max(iso[i], 0), HK_MAX_ISO - 1) : 0;
         double ialphaD_h = kC * sqrt(tab.mass[k] / (2.0 * kKB * T));
-        double Sh = S0[i] / tab.Q[k] * tab.ab[k] * tab.ss[k];
+        double Sh = form == HK_FORM_SUPERLINE ? S0[i] * tab.ab[k] * tab.ss[k]
+                                              : S0[i] / tab.Q[k] * tab.ab[k] * tab.ss[k];
         double key = nu0[i];
-        double nu = fma(delta[i], P, key);
+        double nu = form == HK_FORM_HITRAN ? fma(delta[i], P, key) : key;
         double ia = ialphaD_h / nu;
         double E = EL[i];
-        double S = Sh * (exp(-E * c / T + E * c / kT0) * (1.0 - exp(-c * nu / T))
-                         / (1.0 - exp(-c * nu / kT0)) * ia);
-        double aL = fma(vy0[i], (P / kPOatm) * pow(kT0 / T, nexp[i]), A[i] / (4.0 * kPi * kC));
+        double S;
+        if (form == HK_FORM_HITRAN)
+            S = Sh * (exp(-E * c / T + E * c / kT0) * (1.0 - exp(-c * nu / T))
+                      / (1.0 - exp(-c * nu / kT0)) * ia);
+        else if (form == HK_FORM_EXOMOL)
+            S = Sh * (exp(-c / T * E) * (1.0 - exp(-c / T * nu)) * ia);   // voigt.h:379
+        else
+            S = Sh * ia;                                                  // voigt.h:407
+        double Pn = form == HK_FORM_HITRAN ? P / kPOatm : P / kPObar;
+        double An = form == HK_FORM_SUPERLINE ? 0.0 : A[i] / (4.0 * kPi * kC);
+        double aL = fma(vy0[i], Pn * pow(kT0 / T, nexp[i]), An);
         double y = aL * ia;
         double hw = cut_mode == 1 ? cut * aL : (cut_mode == 2 ? cut / ia : cut);
         if (nu == 0.0) { S = 0.0; ia = 0.0; y = 0.0; hw = -1.0; }
@@ -131,6 +141,8 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
                  (long long)st->nlines, (long long)L->nlines);
     HK_CHECK_ARG(L->niso >= 1 && L->niso <= HK_MAX_ISO, "hk_line_params: niso %d out of 1..%d",
                  L->niso, HK_MAX_ISO);
+    HK_CHECK_ARG(L->form >= HK_FORM_HITRAN && L->form <= HK_FORM_SUPERLINE,
+                 "hk_line_params: form %d", L->form);
     HK_CHECK_ARG(tp->T > 0.0, "hk_line_params: T must be > 0");
     HK_CHECK_ARG(cut->cut_mode >= 0 && cut->cut_mode <= 2, "hk_line_params: cut_mode %d",
                  cut->cut_mode);
@@ -156,7 +168,7 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
         tab.Q[k] = used ? tp->Q[k] : 1.0;
     }
     hipLaunchKernelGGL(line_params_kernel, dim3((unsigned)nb), dim3(256), 0, s, L->nu, L->S,
-                       L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, n, tp->T, tp->P,
+                       L->EL, L->A, L->delta, L->vy, L->n, L->iso, tab, L->form, n, tp->T, tp->P,
                        cut->cut_mode, cutv, st->nu, st->S, st->ialphaD, st->y, st->hw, st->ey,
                        st->blk_max, st->radius + 2, st->radius + 2 + nb);
     HK_LAUNCH_CHECK();

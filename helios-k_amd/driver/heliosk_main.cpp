@@ -69,6 +69,8 @@ struct Isotopologue {
 
 struct Molecule {
     int dataBase = 0, id = 0, npfcol = 0, nFiles = 0;
+    int nStates = 0, ntcol = 0, version = 0;
+    double defaultL = 0.0, defaultn = 0.0;   // ExoMol-family widths (ISO.h:188-204)
     std::string mName;
     std::vector<Isotopologue> iso;
     std::vector<long long> NL;
@@ -328,17 +330,50 @@ bool init_species(Molecule &m, const Param &p, const std::string &species) {
     m.fileLimit.resize(m.nFiles + 1);
     for (int i = 0; i <= m.nFiles; ++i)
         if (std::fscanf(f, "%d", &m.fileLimit[i]) != 1) return bad("Line file limits");
+    skip_line(f);
+    skip_line(f);          // "#ExoMol :"
+    if (!expect_key(f, "Number of states =") || std::fscanf(f, "%d", &m.nStates) != 1)
+        return bad("Number of states");
+    if (!expect_key(f, "Number of columns in transition files =") ||
+        std::fscanf(f, "%d", &m.ntcol) != 1)
+        return bad("Number of columns in transition files");
+    if (!expect_key(f, "Default value of Lorentzian half-width for all lines =") ||
+        std::fscanf(f, "%lf", &m.defaultL) != 1)
+        return bad("Default value of Lorentzian half-width for all lines");
+    if (!expect_key(f, "Default value of temperature exponent for all lines =") ||
+        std::fscanf(f, "%lf", &m.defaultn) != 1)
+        return bad("Default value of temperature exponent for all lines");
+    if (!expect_key(f, "Version =") || std::fscanf(f, "%d", &m.version) != 1) return bad("Version");
     std::fclose(f);
+    // line-file names (ISO.h:125-175); the reader appends "bin"
     for (int i = 0; i < m.nFiles; ++i) {
         char nm[1200];
-        if (m.nFiles > 1)
-            std::snprintf(nm, sizeof(nm), "%s%s_%05d-%05d.", p.path.c_str(), m.mName.c_str(),
-                          m.fileLimit[i], m.fileLimit[i + 1]);
+        const bool split = m.nFiles > 1 && (m.dataBase == 0 || m.dataBase == 1 ||
+                                            m.dataBase == 2 || m.dataBase == 3);
+        const char *sep = (m.dataBase == 2 || m.dataBase == 3) ? "__" : "_";
+        const char *dash = m.dataBase == 3 ? "_" : "-";
+        if (split)
+            std::snprintf(nm, sizeof(nm), "%s%s%s%05d%s%05d.", p.path.c_str(), m.mName.c_str(), sep,
+                          m.fileLimit[i], dash, m.fileLimit[i + 1]);
         else
             std::snprintf(nm, sizeof(nm), "%s%s.", p.path.c_str(), m.mName.c_str());
         m.dataFilename.push_back(nm);
     }
     return true;
+}
+
+// Line form and record length of each database (heliosk.cu:853-873):
+// 0/1/3 HITRAN-style 68-byte records, 2 ExoMol (nu, S, E'', A), 20 ExoMol
+// super-lines (nu, S), 30/31/32 Kurucz / NIST / VALD (nu, S, E'', A, Gamma_nat).
+int database_form(int db) {
+    if (db == 0 || db == 1 || db == 3) return HK_FORM_HITRAN;
+    if (db == 2 || db == 30 || db == 31 || db == 32) return HK_FORM_EXOMOL;
+    if (db == 20) return HK_FORM_SUPERLINE;
+    return -1;
+}
+
+int database_doubles(int db) {
+    return db == 2 ? 4 : (db == 20 ? 2 : 5);
 }
 
 // Partition function at T: linear interpolation in a 2- or 3-column table
@@ -458,6 +493,43 @@ bool read_hitran_bin(const std::string &file, long long NL, const Molecule &m, c
     return true;
 }
 
+// ExoMol-family binary records (host.h:656-667 reads them, voigt.h:484-578
+// decodes them): per line nd doubles -- nu, S [, E'', A [, Gamma_nat]].
+// Widths are the species' defaults (vy = defaultL gammaF, n = defaultn),
+// there is no pressure shift and one isotopologue.
+bool read_exomol_bin(const std::string &file, long long NL, int nd, const Molecule &m,
+                     const Param &p, HostLines &L) {
+    FILE *f = std::fopen(file.c_str(), "rb");
+    if (!f) {
+        std::printf("Error: line list file not found: %s\n", file.c_str());
+        return false;
+    }
+    constexpr long long kChunk = 1 << 20;
+    std::vector<double> buf((size_t)nd * kChunk);
+    for (long long done = 0; done < NL;) {
+        const long long k = std::min(kChunk, NL - done);
+        if (std::fread(buf.data(), sizeof(double) * nd, (size_t)k, f) != (size_t)k) {
+            std::printf("Error: line list file too short: %s\n", file.c_str());
+            std::fclose(f);
+            return false;
+        }
+        for (long long r = 0; r < k; ++r) {
+            const double *v = buf.data() + r * nd;
+            L.nu.push_back(v[0]);
+            L.S.push_back(v[1]);
+            L.EL.push_back(nd > 2 ? v[2] : 0.0);
+            L.A.push_back(nd > 4 ? v[3] + v[4] : (nd > 2 ? v[3] : 0.0));
+            L.delta.push_back(0.0);
+            L.vy.push_back(m.defaultL * p.gammaF);
+            L.n.push_back(m.defaultn);
+            L.iso.push_back(0);
+        }
+        done += k;
+    }
+    std::fclose(f);
+    return true;
+}
+
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
@@ -522,7 +594,11 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
         const std::string fn = m.dataFilename[fi] + "bin";
         std::printf("Reading Line file %d of %d: %s\n", fi, fi1 - 1, fn.c_str());
         std::printf("Number of lines: %lld\n", m.NL[fi]);
-        if (!read_hitran_bin(fn, m.NL[fi], m, p, H)) return false;
+        const int form = database_form(m.dataBase);
+        const bool ok = form == HK_FORM_HITRAN
+                            ? read_hitran_bin(fn, m.NL[fi], m, p, H)
+                            : read_exomol_bin(fn, m.NL[fi], database_doubles(m.dataBase), m, p, H);
+        if (!ok) return false;
     }
     const int64_t nl = (int64_t)H.nu.size();
     // lines sorted by centre (stable: ties keep file order)
@@ -562,7 +638,8 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
     L.nlines = nl;
     L.nu = b; L.S = b + nl; L.EL = b + 2 * nl; L.A = b + 3 * nl; L.delta = b + 4 * nl;
     L.vy = b + 5 * nl; L.n = b + 6 * nl; L.iso = iso_d;
-    L.niso = (int)m.iso.size();
+    L.niso = database_form(m.dataBase) == HK_FORM_HITRAN ? (int)m.iso.size() : 1;
+    L.form = database_form(m.dataBase);
     double mdelta = 0.0;
     for (double d : H.delta) mdelta = std::max(mdelta, std::fabs(d));
     L.max_abs_delta = mdelta;
@@ -726,9 +803,8 @@ bool run(int argc, char **argv) {
     double meanMass = 0.0;
     for (size_t k = 0; k < spNames.size(); ++k) {
         if (!init_species(mols[k], p, spNames[k])) { std::fclose(info); return false; }
-        if (mols[k].dataBase != 0) {
-            std::printf("Error: Database %d is not supported (HITRAN/HITEMP binary files, "
-                        "Database = 0, only)\n", mols[k].dataBase);
+        if (database_form(mols[k].dataBase) < 0) {
+            std::printf("Error: Database %d is not supported\n", mols[k].dataBase);
             std::fclose(info);
             return false;
         }

@@ -10,6 +10,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libheliosk.so")
 
 HK_OK = 0
+HK_FORM_HITRAN, HK_FORM_EXOMOL, HK_FORM_SUPERLINE = 0, 1, 2
 HK_VOIGT_FP64 = 0
 HK_VOIGT_FP32C = 1
 HK_MAX_ISO = 16
@@ -25,7 +26,7 @@ P_d = ctypes.POINTER(ctypes.c_double)
 class Lines(ctypes.Structure):
     _fields_ = [("nlines", c_i64),
                 ("nu", c_vp), ("S", c_vp), ("EL", c_vp), ("A", c_vp), ("delta", c_vp),
-                ("vy", c_vp), ("n", c_vp), ("iso", c_vp), ("niso", c_int),
+                ("vy", c_vp), ("n", c_vp), ("iso", c_vp), ("niso", c_int), ("form", c_int),
                 ("iso_mass", c_d * HK_MAX_ISO), ("iso_ab", c_d * HK_MAX_ISO),
                 ("iso_sscale", c_d * HK_MAX_ISO), ("max_abs_delta", c_d)]
 

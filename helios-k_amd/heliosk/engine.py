@@ -12,12 +12,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (HK_VOIGT_FP32C, HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineState, Lines,
-                   Spectrum, TP, Tune, check, lib)
+from ._lib import (HK_FORM_EXOMOL, HK_FORM_HITRAN, HK_FORM_SUPERLINE, HK_VOIGT_FP32C,
+                   HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineState, Lines, Spectrum, TP, Tune,
+                   check, lib)
 
 NA = 6.0221412927e23          # define.h:42
 
-__all__ = ["Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
+__all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
            "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
@@ -44,10 +45,12 @@ def make_grid(numin, numax, dnu):
 
 
 class Species:
-    """Raw line list of one species on the device, sorted by nu (stable)."""
+    """Raw line list of one species on the device, sorted by nu (stable).
+    form: HK_FORM_HITRAN, HK_FORM_EXOMOL or HK_FORM_SUPERLINE (how the
+    strengths and widths depend on T and P)."""
 
     def __init__(self, nu, S, EL, vy, n, A=None, delta=None, iso=None, iso_mass=(18.0 / NA,),
-                 iso_ab=(1.0,), iso_sscale=(1.0,), name="species"):
+                 iso_ab=(1.0,), iso_sscale=(1.0,), name="species", form=HK_FORM_HITRAN):
         nu = np.asarray(nu, dtype=np.float64)
         order = np.argsort(nu, kind="stable")
         dev = _dev()
@@ -58,6 +61,7 @@ class Species:
             return torch.from_numpy(np.ascontiguousarray(a[order])).to(dev)
 
         self.name = name
+        self.form = int(form)
         self.order = order
         self.nu = up(nu); self.S = up(S); self.EL = up(EL); self.vy = up(vy); self.n = up(n)
         self.A = up(A); self.delta = up(delta)
@@ -90,6 +94,7 @@ class Species:
         L.n = self.n.data_ptr()
         L.iso = self.iso.data_ptr() if self.iso is not None else None
         L.niso = len(self.iso_mass)
+        L.form = self.form
         for k in range(L.niso):
             L.iso_mass[k] = self.iso_mass[k]; L.iso_ab[k] = self.iso_ab[k]
             L.iso_sscale[k] = self.iso_sscale[k]

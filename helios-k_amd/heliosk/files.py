@@ -59,9 +59,10 @@ def write_partition(path, T, Q):
 
 
 def write_species_param(path, name, isotopologues, nlines, limits, database=0, molecule=1,
-                        npfcol=2):
+                        npfcol=2, default_gamma=0.0, default_n=0.0):
     """isotopologues: [(cid, abundance, Q296, g, molar mass, partition file)];
-    nlines: lines per file; limits: nfiles+1 file limits (cm^-1)."""
+    nlines: lines per file; limits: nfiles+1 file limits (cm^-1);
+    default_gamma / default_n: the ExoMol-family line widths (ISO.h:188-204)."""
     with open(path, "w") as f:
         f.write("Database = %d\n" % database)
         f.write("Molecule number = %d\n" % molecule)
@@ -81,9 +82,34 @@ def write_species_param(path, name, isotopologues, nlines, limits, database=0, m
         f.write("#ExoMol :\n")
         f.write("Number of states = 0\n")
         f.write("Number of columns in transition files = 0\n")
-        f.write("Default value of Lorentzian half-width for all lines = 0\n")
-        f.write("Default value of temperature exponent for all lines = 0\n")
+        f.write("Default value of Lorentzian half-width for all lines = %.12g\n" % default_gamma)
+        f.write("Default value of temperature exponent for all lines = %.12g\n" % default_n)
         f.write("Version = 0\n")
+
+
+# doubles per line of the ExoMol-family binary files (heliosk.cu:853-873)
+EXOMOL_DOUBLES = {2: 4, 20: 2, 30: 5, 31: 5, 32: 5}
+
+
+def write_exomol_bin(path, database, nu, S, EL=None, A=None, gamma_nat=None):
+    """Binary line file of an ExoMol-family database: per line nu, S
+    [, E'', A [, Gamma_nat]] as little-endian doubles (the layout
+    prepareExomol.cpp writes and host.h:656-667 reads)."""
+    nd = EXOMOL_DOUBLES[database]
+    cols = [nu, S, EL, A, gamma_nat][:nd]
+    rec = np.empty((len(nu), nd), dtype="<f8")
+    for k, c in enumerate(cols):
+        rec[:, k] = c
+    rec.tofile(path)
+
+
+def species_param_name(base, database, nfiles, lo, hi):
+    """Line-file stem of file [lo, hi) of a species (ISO.h:125-175)."""
+    if nfiles > 1 and database in (0, 1, 2, 3):
+        sep = "__" if database in (2, 3) else "_"
+        dash = "_" if database == 3 else "-"
+        return "%s%s%05d%s%05d." % (base, sep, lo, dash, hi)
+    return base + "."
 
 
 def write_param_dat(path, **overrides):

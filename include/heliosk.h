@@ -53,18 +53,31 @@ enum {
     HK_VOIGT_FP32C = 1          /* FP32 profile, compensated FP32 tile sums */
 };
 
+/* Line-list forms: how the file strength and widths become S(T), alpha_L(T,P).
+ *  HK_FORM_HITRAN    HITRAN/HITEMP (S2_kernel, voigt.h:331-363): Boltzmann and
+ *                    stimulated-emission factors relative to T0 = 296 K,
+ *                    pressure shift delta P, P in atm.
+ *  HK_FORM_EXOMOL    ExoMol / Kurucz / NIST / VALD (Sf_kernel, voigt.h:365-397):
+ *                    exp(-c2 E/T) (1 - exp(-c2 nu/T)), no shift, P / 0.986923.
+ *  HK_FORM_SUPERLINE ExoMol super-lines (SfSuper_kernel, voigt.h:399-420):
+ *                    S as stored (no Q), alpha_L = vy (P/0.986923) (T0/T)^n. */
+enum { HK_FORM_HITRAN = 0, HK_FORM_EXOMOL = 1, HK_FORM_SUPERLINE = 2 };
+
 /* Raw line list of one species, device SoA, sorted by nu ascending.
  * Per line: nu [cm^-1], S = S_file (the strength as stored in the line
  * file), EL [cm^-1], A [s^-1], delta [cm^-1/atm], vy = ((1-q) g_air +
  * q g_self) * gammaF [cm^-1/atm], n (temperature exponent), iso (index
  * into the isotopologue tables; NULL means 0 for every line).
- * Strength at (T,P) follows readFile (host.h:641-644) and S2_kernel:
- * S_h = S_file / Q[iso] * iso_ab[iso] * iso_sscale[iso]. */
+ * Strength at (T,P) follows readFile (host.h:641-644) and the form's
+ * kernel: S_h = S_file / Q[iso] * iso_ab[iso] * iso_sscale[iso] (no Q for
+ * HK_FORM_SUPERLINE).  A is the Einstein A (ExoMol-family lists: A plus the
+ * natural-broadening gamma), added to alpha_L as A / (4 pi c). */
 typedef struct {
     int64_t nlines;
     const double *nu, *S, *EL, *A, *delta, *vy, *n;
     const int32_t *iso;
     int niso;
+    int form;                       /* HK_FORM_* */
     double iso_mass[HK_MAX_ISO];    /* mass of one molecule [g] (M / N_A) */
     double iso_ab[HK_MAX_ISO];
     double iso_sscale[HK_MAX_ISO];

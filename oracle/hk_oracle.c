@@ -165,7 +165,7 @@ void hko_line_params(int64_t nl,
                      const double *nexp, const int32_t *iso,
                      const double *iso_mass, const double *iso_ab,
                      const double *iso_sscale, const double *iso_Q,
-                     double T, double P, int cutMode, double cut,
+                     double T, double P, int cutMode, double cut, int form,
                      double *nu_o, double *S_o, double *ia_o, double *y_o,
                      double *hw_o)
 {
@@ -173,17 +173,37 @@ void hko_line_params(int64_t nl,
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nl; ++i) {
         int k = iso ? iso[i] : 0;
-        /* readFile, host.h:641-644 */
+        /* readFile, host.h:641-644; L_kernelExomol / L_kernelExomolSuper,
+           voigt.h:484-531 (super-lines: no partition function) */
         double ialphaD_h = HKO_C * sqrt(iso_mass[k] / (2.0 * HKO_KB * T));
-        double Sh = S0[i] / iso_Q[k] * iso_ab[k] * iso_sscale[k];
-        /* S2_kernel, voigt.h:338-356 */
-        double nu = fma(delta[i], P, nu0[i]);
-        double ialphaD = ialphaD_h / nu;
+        double Sh = form == 2 ? S0[i] * iso_ab[k] * iso_sscale[k]
+                              : S0[i] / iso_Q[k] * iso_ab[k] * iso_sscale[k];
+        double nu, S, alphaL;
         double E = EL[i];
-        double S = Sh * (exp(-E * c / T + E * c / HKO_T0) * (1.0 - exp(-c * nu / T))
-                         / (1.0 - exp(-c * nu / HKO_T0)) * ialphaD);
-        double alphaL = fma(vy0[i], (P / HKO_POATM) * pow(HKO_T0 / T, nexp[i]),
-                            A[i] / (4.0 * M_PI * HKO_C));
+        if (form == 0) {
+            /* S2_kernel, voigt.h:338-356 */
+            nu = fma(delta[i], P, nu0[i]);
+            double ia = ialphaD_h / nu;
+            S = Sh * (exp(-E * c / T + E * c / HKO_T0) * (1.0 - exp(-c * nu / T))
+                      / (1.0 - exp(-c * nu / HKO_T0)) * ia);
+            alphaL = fma(vy0[i], (P / HKO_POATM) * pow(HKO_T0 / T, nexp[i]),
+                         A[i] / (4.0 * M_PI * HKO_C));
+        } else if (form == 1) {
+            /* Sf_kernel, voigt.h:371-384, c1 = h c / (kB T), T1 = T0 / T */
+            double c1 = HKO_H * HKO_C / (HKO_KB * T);
+            nu = nu0[i];
+            double ia = ialphaD_h / nu;
+            S = Sh * (exp(-c1 * E) * (1.0 - exp(-c1 * nu)) * ia);
+            alphaL = fma(vy0[i], (P / HKO_POBAR) * pow(HKO_T0 / T, nexp[i]),
+                         A[i] / (4.0 * M_PI * HKO_C));
+        } else {
+            /* SfSuper_kernel, voigt.h:405-414 */
+            nu = nu0[i];
+            double ia = ialphaD_h / nu;
+            S = Sh * ia;
+            alphaL = vy0[i] * ((P / HKO_POBAR) * pow(HKO_T0 / T, nexp[i]));
+        }
+        double ialphaD = ialphaD_h / nu;
         double y = alphaL * ialphaD;
         double hw;
         if (cutMode == 1) hw = cut * alphaL;
@@ -295,7 +315,7 @@ void hko_voigt_grad(int64_t nl,
                     const double *nexp, const int32_t *iso,
                     const double *iso_mass, const double *iso_ab,
                     const double *iso_sscale, const double *iso_Q,
-                    double T, double P, int cutMode, double cut,
+                    double T, double P, int cutMode, double cut, int form,
                     double numin, double dnu, int64_t Nx, const double *gK,
                     double *gS0, double *gvy0, double *gTP, int nthreads)
 {
@@ -310,31 +330,34 @@ void hko_voigt_grad(int64_t nl,
     for (int64_t i = 0; i < nl; ++i) {
         int k = iso ? iso[i] : 0;
         double ialphaD_h = HKO_C * sqrt(iso_mass[k] / (2.0 * HKO_KB * T));
-        double F0 = 1.0 / iso_Q[k] * iso_ab[k] * iso_sscale[k];
-        double nu = fma(delta[i], P, nu0[i]);
+        double F0 = form == 2 ? iso_ab[k] * iso_sscale[k]
+                              : 1.0 / iso_Q[k] * iso_ab[k] * iso_sscale[k];
+        double dlt = form == 0 ? delta[i] : 0.0;
+        double nu = fma(dlt, P, nu0[i]);
         if (nu == 0.0) { gS0[i] = 0.0; gvy0[i] = 0.0; continue; }
         double ia = ialphaD_h / nu;
         double E = EL[i];
-        double B = exp(-E * c / T + E * c / HKO_T0);
         double eT = exp(-c * nu / T), e0 = exp(-c * nu / HKO_T0);
-        double Rr = (1.0 - eT) / (1.0 - e0);
+        double B = form == 0 ? exp(-E * c / T + E * c / HKO_T0) : (form == 1 ? exp(-c / T * E) : 1.0);
+        double Rr = form == 0 ? (1.0 - eT) / (1.0 - e0) : (form == 1 ? 1.0 - eT : 1.0);
         double F = F0 * B * Rr * ia;            /* S = S0 * F */
         double S = S0[i] * F;
         double pw = pow(HKO_T0 / T, nexp[i]);
-        double aLp = vy0[i] * P * pw;           /* pressure part of alpha_L */
-        double aL = aLp + A[i] / (4.0 * M_PI * HKO_C);
+        double Pscale = form == 0 ? 1.0 / HKO_POATM : 1.0 / HKO_POBAR;
+        double aLp = vy0[i] * P * Pscale * pw;  /* pressure part of alpha_L */
+        double aL = aLp + (form == 2 ? 0.0 : A[i] / (4.0 * M_PI * HKO_C));
         double y = aL * ia;
         double hw = cutMode == 1 ? cut * aL : (cutMode == 2 ? cut / ia : cut);
         /* log-derivatives */
-        double dlnR_dT = -eT * (c * nu / (T * T)) / (1.0 - eT);
+        double dlnR_dT = form == 2 ? 0.0 : -eT * (c * nu / (T * T)) / (1.0 - eT);
         double dlnR_dnu = eT * (c / T) / (1.0 - eT) - e0 * (c / HKO_T0) / (1.0 - e0);
-        double dlnS_dT = c * E / (T * T) + dlnR_dT - 0.5 / T;
-        double dlnS_dP = delta[i] * (dlnR_dnu - 1.0 / nu);
-        double dia_dT = -0.5 * ia / T, dia_dP = -ia * delta[i] / nu;
-        double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * pw;
+        double dlnS_dT = (form == 2 ? 0.0 : c * E / (T * T)) + dlnR_dT - 0.5 / T;
+        double dlnS_dP = dlt * (dlnR_dnu - 1.0 / nu);
+        double dia_dT = -0.5 * ia / T, dia_dP = -ia * dlt / nu;
+        double daL_dT = -nexp[i] / T * aLp, daL_dP = vy0[i] * Pscale * pw;
         double dy_dT = daL_dT * ia + aL * dia_dT;
         double dy_dP = daL_dP * ia + aL * dia_dP;
-        double dy_dvy = P * pw * ia;
+        double dy_dvy = P * Pscale * pw * ia;
         int64_t jlo, jhi;
         hko_window(nu, hw, numin, dnu, Nx, &jlo, &jhi);
         double sre = 0.0, sdy = 0.0, sT = 0.0, sP = 0.0;
@@ -353,7 +376,7 @@ void hko_voigt_grad(int64_t nl,
                 dRe_dy = 2.0 * (x * im + y * re) - 2.0 * HKO_ISQRTPI;
             }
             double dx_dT = d * dia_dT;
-            double dx_dP = -delta[i] * ia + d * dia_dP;
+            double dx_dP = -dlt * ia + d * dia_dP;
             sre += g * re;
             sdy += g * dRe_dy;
             sT += g * (re * dlnS_dT + dRe_dx * dx_dT + dRe_dy * dy_dT);

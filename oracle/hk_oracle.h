@@ -43,7 +43,7 @@ void hko_line_params(int64_t nl,
                      const double *nexp, const int32_t *iso,
                      const double *iso_mass, const double *iso_ab,
                      const double *iso_sscale, const double *iso_Q,
-                     double T, double P, int cutMode, double cut,
+                     double T, double P, int cutMode, double cut, int form,
                      double *nu, double *S, double *ialphaD, double *y,
                      double *hw);
 
@@ -66,7 +66,7 @@ void hko_voigt_grad(int64_t nl,
                     const double *nexp, const int32_t *iso,
                     const double *iso_mass, const double *iso_ab,
                     const double *iso_sscale, const double *iso_Q,
-                    double T, double P, int cutMode, double cut,
+                    double T, double P, int cutMode, double cut, int form,
                     double numin, double dnu, int64_t Nx, const double *gK,
                     double *gS0, double *gvy0, double *gTP, int nthreads);
 

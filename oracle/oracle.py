@@ -38,12 +38,12 @@ def lib():
         L.hko_w916.argtypes = [c_d, c_d, c_d, _D, _D]
         L.hko_wcf.argtypes = [c_d, c_d, c_int, _D, _D]
         L.hko_line_params.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
-                                      + [c_d, c_d, c_int, c_d] + [_D] * 5)
+                                      + [c_d, c_d, c_int, c_d, c_int] + [_D] * 5)
         L.hko_window.argtypes = [c_d, c_d, c_d, c_d, c_i64, _I64, _I64]
         L.hko_voigt_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d,
                                                             c_i64, _I64, _D, c_int])
         L.hko_voigt_grad.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
-                                     + [c_d, c_d, c_int, c_d, c_d, c_d, c_i64, _D,
+                                     + [c_d, c_d, c_int, c_d, c_int, c_d, c_d, c_i64, _D,
                                         _D, _D, _D, c_int])
         L.hko_bin_segments.argtypes = [c_d, c_d, c_d, c_i64, c_int, _I64]
         L.hko_sort_segments.argtypes = [_D, c_int, _I64]
@@ -94,8 +94,9 @@ def _iso_arrays(lines):
     return iso, tabs
 
 
-def line_params(lines, T, P, cut_mode, cut):
-    """Line state at (T, P[atm]): dict nu, S, ialphaD, y, hw."""
+def line_params(lines, T, P, cut_mode, cut, form=0):
+    """Line state at (T, P[atm]): dict nu, S, ialphaD, y, hw.  form: 0 HITRAN
+    (S2_kernel), 1 ExoMol family (Sf_kernel), 2 super-lines (SfSuper_kernel)."""
     nl = len(lines["nu"])
     cols = [_f64(lines[k]) for k in ("nu", "S", "EL", "A", "delta", "vy", "n")]
     iso, tabs = _iso_arrays(lines)
@@ -103,7 +104,8 @@ def line_params(lines, T, P, cut_mode, cut):
     lib().hko_line_params(nl, *[_p(c) for c in cols],
                           None if iso is None else _p(iso, _I32),
                           *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
-                          float(cut), *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
+                          float(cut), int(form),
+                          *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
     return out
 
 
@@ -139,7 +141,7 @@ def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
     return K
 
 
-def voigt_grad(lines, T, P, cut_mode, cut, numin, dnu, Nx, gK, nthreads=0):
+def voigt_grad(lines, T, P, cut_mode, cut, numin, dnu, Nx, gK, nthreads=0, form=0):
     """(dL/dT, dL/dP, dL/dS_i, dL/dvy_i) for L = sum_j gK_j K_j."""
     nl = len(lines["nu"])
     cols = [_f64(lines[k]) for k in ("nu", "S", "EL", "A", "delta", "vy", "n")]
@@ -149,7 +151,7 @@ def voigt_grad(lines, T, P, cut_mode, cut, numin, dnu, Nx, gK, nthreads=0):
     lib().hko_voigt_grad(nl, *[_p(c) for c in cols],
                          None if iso is None else _p(iso, _I32),
                          *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
-                         float(cut), float(numin), float(dnu), int(Nx), _p(gK),
+                         float(cut), int(form), float(numin), float(dnu), int(Nx), _p(gK),
                          _p(gS), _p(gv), _p(gTP), int(nthreads))
     return gTP[0], gTP[1], gS, gv
 

@@ -17,11 +17,11 @@ def random_lines(n, seed=0, numin=1000.0, numax=1100.0, delta=0.0, A=0.0, niso=1
                 iso_sscale=np.ones(niso), iso_Q=np.array([174.58, 176.05, 1052.14, 900.0])[:niso])
 
 
-def species_from(d):
+def species_from(d, form=0):
     from heliosk.engine import Species
     return Species(d["nu"], d["S"], d["EL"], d["vy"], d["n"], A=d["A"], delta=d["delta"],
                    iso=d["iso"], iso_mass=tuple(d["iso_mass"]), iso_ab=tuple(d["iso_ab"]),
-                   iso_sscale=tuple(d["iso_sscale"]))
+                   iso_sscale=tuple(d["iso_sscale"]), form=form)
 
 
 def state_numpy(st):

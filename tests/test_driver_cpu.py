@@ -39,6 +39,26 @@ def test_hitran_record_roundtrip(tmp_path):
     assert raw[:4] == b" 12\0"
 
 
+@pytest.mark.parametrize("database, nd", [(2, 4), (20, 2), (30, 5), (31, 5), (32, 5)])
+def test_exomol_record_layout(tmp_path, database, nd):
+    rng = np.random.default_rng(database)
+    cols = [rng.uniform(0, 1, 7) for _ in range(5)]
+    p = tmp_path / "x.bin"
+    F.write_exomol_bin(str(p), database, *cols)
+    raw = np.fromfile(p, dtype="<f8")
+    assert raw.size == 7 * nd
+    np.testing.assert_array_equal(raw.reshape(7, nd), np.stack(cols[:nd], axis=1))
+
+
+def test_line_file_names():
+    # ISO.h:125-175
+    assert F.species_param_name("01_hit16", 0, 2, 0, 250) == "01_hit16_00000-00250."
+    assert F.species_param_name("1H2-16O__BT2", 2, 16, 250, 500) == "1H2-16O__BT2__00250-00500."
+    assert F.species_param_name("CO2_cdsd", 3, 3, 500, 1000) == "CO2_cdsd__00500_01000."
+    assert F.species_param_name("x", 20, 3, 0, 10) == "x."
+    assert F.species_param_name("x", 2, 1, 0, 10) == "x."
+
+
 def _setup(tmp_path, **kw):
     F.write_param_dat(str(tmp_path / "param.dat"), **kw)
 

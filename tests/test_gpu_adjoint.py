@@ -38,6 +38,24 @@ def test_adjoint_vs_oracle(E, T, P, cm, cut, delta):
     assert np.max(np.abs(gv.cpu().numpy() - ov)) <= 1e-10 * v
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_adjoint_exomol_forms_vs_oracle(E, form):
+    import torch
+    d = random_lines(400, seed=9, numin=995.0, numax=1015.0, A=5.0e7, EL=(0.0, 2000.0))
+    grid = E.make_grid(1000.0, 1010.0, 0.002)
+    gK = np.random.default_rng(3).uniform(0.0, 1.0, grid.nx)
+    sp = species_from(d, form=form)
+    T, P = 900.0, 0.4
+    gTP, gS, gv = E.adjoint(sp, T, P, torch.from_numpy(gK).cuda(), grid, Q=d["iso_Q"],
+                            cut_mode=0, cut=5.0)
+    oT, oP, oS, ov = O.voigt_grad(d, T, P, 0, 5.0, grid.numin, grid.dnu, grid.nx, gK, form=form)
+    gTP = gTP.cpu().numpy()
+    assert abs(gTP[0] - oT) <= 1e-10 * abs(oT)
+    assert abs(gTP[1] - oP) <= 1e-10 * abs(oP)
+    assert np.max(np.abs(gS.cpu().numpy() - oS)) <= 1e-10 * np.max(np.abs(oS))
+    assert np.max(np.abs(gv.cpu().numpy() - ov)) <= 1e-10 * np.max(np.abs(ov))
+
+
 def test_adjoint_vs_central_differences(E):
     import torch
     d = random_lines(60, seed=8, numin=998.0, numax=1012.0, EL=(0.0, 500.0))

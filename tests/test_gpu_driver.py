@@ -177,3 +177,95 @@ def test_driver_fp32c_precision_key(tmp_path):
     e = rel_err(Kd, Ko)
     print("driver FP32C K vs oracle %.3e" % e)
     assert 0.0 < e <= 1e-5
+
+
+def _exomol_species(tmp, name, database, nfiles_limits, n, seed, T_table, ab=1.0, mass=18.010565,
+                    gamma=0.07, nexp=0.5, npfcol=3):
+    """ExoMol-family inputs: <name>.param with the #ExoMol block, a partition
+    table (npfcol columns) and binary line files of 2, 4 or 5 doubles per line,
+    named as ISO.h:125-175 names them."""
+    rng = np.random.default_rng(seed)
+    nd = F.EXOMOL_DOUBLES[database]
+    lim = nfiles_limits
+    nfiles = len(lim) - 1
+    cols = dict(nu=[], S=[], EL=[], A=[], gN=[])
+    counts = []
+    for k in range(nfiles):
+        nu = np.sort(rng.uniform(lim[k], lim[k + 1], n))
+        c = dict(nu=nu, S=10 ** rng.uniform(-24, -19, n), EL=rng.uniform(0, 5000, n),
+                 A=rng.uniform(0, 5e7, n) * (rng.uniform(size=n) < 0.3),
+                 gN=rng.uniform(0, 2e7, n) * (rng.uniform(size=n) < 0.3))
+        stem = F.species_param_name(name, database, nfiles, lim[k], lim[k + 1])
+        F.write_exomol_bin(str(tmp / (stem + "bin")), database, c["nu"], c["S"], c["EL"], c["A"],
+                           c["gN"])
+        for key in cols:
+            cols[key].append(c[key] if nd > 2 or key in ("nu", "S") else np.zeros(n))
+        counts.append(n)
+    Q = 1.0 + 0.3 * T_table ** 1.4
+    with open(tmp / ("%s.pf" % name), "w") as f:
+        for t, q in zip(T_table, Q):
+            f.write(("%8.1f %16.6f %d\n" % (t, q, 1)) if npfcol == 3 else ("%8.1f %16.6f\n" % (t, q)))
+    F.write_species_param(str(tmp / (name + ".param")), name,
+                          [("0", ab, 0.0, 0, mass, "%s.pf" % name)], counts, lim,
+                          database=database, npfcol=npfcol, default_gamma=gamma, default_n=nexp)
+    lines = {k: np.concatenate(v) for k, v in cols.items()}
+    if nd < 5:
+        lines["gN"] = np.zeros_like(lines["nu"])
+    if nd < 4:
+        lines["A"] = np.zeros_like(lines["nu"])
+        lines["EL"] = np.zeros_like(lines["nu"])
+    return dict(lines=lines, Q=Q, ab=ab, mass=mass, gamma=gamma, nexp=nexp)
+
+
+def oracle_exomol_K(sp, form, T, P, gammaF, cut, numin, dnu, nx):
+    """Sf_kernel / SfSuper_kernel (voigt.h:365-410) with the L_kernel* line
+    set-up (voigt.h:484-578): Units = 0 scaling of one isotopologue, whose
+    abundance-weighted molar mass is the mean mass (Ab m / meanMass = 1 and
+    Sscale m / meanMass = 1 / Ab)."""
+    L = sp["lines"]
+    nl = L["nu"].size
+    q = q_at_table(T, sp["Q"])
+    d = dict(nu=L["nu"], S=L["S"], EL=L["EL"], A=L["A"] + L["gN"], delta=np.zeros(nl),
+             vy=np.full(nl, sp["gamma"] * gammaF), n=np.full(nl, sp["nexp"]), iso=None,
+             iso_mass=np.array([sp["mass"] / NA]), iso_ab=np.array([1.0]),
+             iso_sscale=np.array([1.0 / sp["ab"]]), iso_Q=np.array([q]))
+    order = np.argsort(d["nu"], kind="stable")
+    d = {k: (v[order] if isinstance(v, np.ndarray) and v.shape == order.shape else v)
+         for k, v in d.items()}
+    st = O.line_params(d, T, P, 0, cut, form=form)
+    return O.voigt_points(st, numin, dnu, nx)
+
+
+def q_at_table(T, Q):
+    return q_at(T, Q)
+
+
+@pytest.mark.parametrize("database, limits, ab, pfile", [
+    (2, [990, 1005, 1020], 1.0, True),       # ExoMol, two files, two pressures
+    (20, [990, 1020], 0.9, False),           # ExoMol super-lines
+    (30, [990, 1020], 1.0, False),           # Kurucz atoms (A + Gamma_nat)
+])
+def test_driver_exomol_family_vs_oracle(tmp_path, database, limits, ab, pfile):
+    name = "1H2-16O__TST"
+    sp = _exomol_species(tmp_path, name, database, limits, 1500, 20 + database, T_TABLE, ab=ab)
+    T, gF, cut = 1400.0, 1.2, 5.0
+    Ps = [0.5, 4.0] if pfile else [2.0]
+    kw = {}
+    if pfile:
+        (tmp_path / "P.txt").write_text("".join("%g\n" % v for v in Ps))
+        kw["PFile"] = "P.txt"
+    F.write_param_dat(str(tmp_path / "param.dat"), name="x", T=T, P=Ps[0], Species_Name=name,
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=2, nbins=NBINS,
+                      gammaF=gF, **kw)
+    out = run_driver(str(tmp_path))
+    nfiles = len(limits) - 1
+    assert "File range 0 to %d" % (nfiles - 1) in out
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    Kf = F.read_out_bin(str(tmp_path / "Out_x.bin"), nx)
+    assert Kf.shape == (len(Ps), nx)
+    form = 2 if database == 20 else 1
+    for iP, P in enumerate(Ps):
+        Ko = oracle_exomol_K(sp, form, T, P, gF, cut, NUMIN, DNU, nx)
+        e = rel_err(Kf[iP].astype(np.float64), Ko)
+        print("database %d, P %g: float32 output vs oracle %.3e" % (database, P, e))
+        assert e <= 1.2e-7

@@ -47,6 +47,24 @@ def test_line_params_vs_oracle(E, case):
     np.testing.assert_allclose(g["hw"], o["hw"], rtol=2e-13, atol=0)
 
 
+@pytest.mark.parametrize("form", [1, 2])
+def test_line_params_exomol_forms_vs_oracle(E, form):
+    """Sf_kernel / SfSuper_kernel semantics (ExoMol family, super-lines),
+    then the plain FP64 sum of the resulting state."""
+    d = random_lines(5000, seed=13, A=2.0e7, EL=(0.0, 20000.0), numin=995.0, numax=1015.0)
+    sp = species_from(d, form=form)
+    T, P = 1700.0, 0.3
+    st = E.line_params(sp, T, P, Q=d["iso_Q"], cut_mode=0, cut=5.0)
+    g = state_numpy(st)
+    o = O.line_params(d, T, P, 0, 5.0, form=form)
+    for k in ("nu", "S", "ialphaD", "y", "hw"):
+        np.testing.assert_allclose(g[k], o[k], rtol=2e-13, atol=0, err_msg=k)
+    grid = E.make_grid(1000.0, 1010.0, 0.001)
+    K = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    Ko = O.voigt_points(o, grid.numin, grid.dnu, grid.nx)
+    assert rel_err(K, Ko) <= 1e-12
+
+
 def _grid(E, numin, numax, dnu):
     return E.make_grid(numin, numax, dnu)
 

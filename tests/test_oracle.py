@@ -120,6 +120,33 @@ def test_line_params_shift_and_natural_width():
     assert st["S"][0] == 0 and st["ialphaD"][0] == 0 and st["hw"][0] < 0
 
 
+def test_line_params_exomol_and_superline_forms():
+    """Sf_kernel (voigt.h:365-385) and SfSuper_kernel (voigt.h:397-410)
+    against the HITRAN form: the ExoMol strength lacks the T0 normalisation
+    (ratio exp(-c2 E/T0) (1 - exp(-c2 nu/T0))), widths scale with P/0.986923
+    bar, centres are unshifted; super-lines carry no Q, no E'' and no A."""
+    T, P = 1234.0, 2.5
+    L = _lines(300, seed=7, delta=-0.003, A=3.0e7)
+    L["iso_Q"] = np.array([321.0])
+    hit = O.line_params(dict(L, delta=np.zeros(300), A=np.zeros(300)), T, P, 0, 25.0, form=0)
+    exo0 = O.line_params(dict(L, A=np.zeros(300)), T, P, 0, 25.0, form=1)
+    exo = O.line_params(L, T, P, 0, 25.0, form=1)
+    sup = O.line_params(L, T, P, 0, 25.0, form=2)
+    for st in (exo, sup):
+        np.testing.assert_array_equal(st["nu"], L["nu"])       # no pressure shift
+        np.testing.assert_allclose(st["ialphaD"], hit["ialphaD"], rtol=1e-15)
+    np.testing.assert_allclose(exo["S"] / hit["S"],
+                               np.exp(-C2 * L["EL"] / T0) * (1 - np.exp(-C2 * L["nu"] / T0)),
+                               rtol=1e-12)
+    np.testing.assert_allclose(exo0["y"] / hit["y"], 1.0 / 0.986923, rtol=1e-14)
+    np.testing.assert_allclose((exo["y"] - exo0["y"]) / exo["ialphaD"],
+                               3.0e7 / (4 * math.pi * 2.99792458e10), rtol=1e-9)
+    # super-lines: S_file * Ab * Sscale / alpha_D, Lorentz width without A
+    np.testing.assert_allclose(sup["S"], L["S"] * 0.9973 * sup["ialphaD"], rtol=1e-15)
+    np.testing.assert_allclose(sup["y"], exo0["y"], rtol=1e-15)
+    np.testing.assert_array_equal(sup["hw"], exo["hw"])
+
+
 # ---------------------------------------------------------------- windows
 def test_window_brute_force():
     rng = np.random.default_rng(3)
@@ -200,8 +227,9 @@ def test_voigt_points_normalisation():
 
 
 # ---------------------------------------------------------------- gradient
-def test_voigt_grad_vs_central_differences():
-    L = _lines(6, seed=4)
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_voigt_grad_vs_central_differences(form):
+    L = _lines(6, seed=4, delta=-0.002 if form == 0 else 0.0, A=1.0e8)
     L["nu"] = np.sort(np.random.default_rng(5).uniform(1000.0, 1001.0, 6))
     L["S"] = np.full(6, 1e-20)
     L["EL"] = np.linspace(0.0, 300.0, 6)        # comparable line contributions
@@ -210,10 +238,10 @@ def test_voigt_grad_vs_central_differences():
     gK = np.random.default_rng(6).uniform(0.0, 1.0, Nx)
 
     def loss(LL, TT, PP):
-        st = O.line_params(LL, TT, PP, 0, 0.8)
+        st = O.line_params(LL, TT, PP, 0, 0.8, form=form)
         return float(np.dot(gK, O.voigt_points(st, numin, dnu, Nx)))
 
-    gT, gP, gS, gv = O.voigt_grad(L, T, P, 0, 0.8, numin, dnu, Nx, gK)
+    gT, gP, gS, gv = O.voigt_grad(L, T, P, 0, 0.8, numin, dnu, Nx, gK, form=form)
     h = 1e-5
     fdT = (loss(L, T * (1 + h), P) - loss(L, T * (1 - h), P)) / (2 * h * T)
     fdP = (loss(L, T, P * (1 + h)) - loss(L, T, P * (1 - h))) / (2 * h * P)
