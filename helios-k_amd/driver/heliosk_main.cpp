@@ -4,8 +4,8 @@
 // (heliosk.cu:160-2360): it reads ./param.dat (host.h:75-300) with the
 // console overrides of host.h:305-386, the species file
 // <pathToData><Species Name>.param (ISO.h:16-219), the partition-function
-// files (host.h:9-73) and the HITRAN/HITEMP binary line files
-// <pathToData><Name>[_lo-hi].bin of 68-byte records (host.h:598-649), and
+// files (host.h:9-73) and the binary line files <pathToData><Name>[_lo-hi].bin
+// (host.h:598-668), and
 // writes the reference's outputs:
 //   Info_<name>.dat                        run parameters and timings
 //   Out_<name>.dat   (doStoreFullK = 1)    "%.20g %.20g" nu, K [T P]  (heliosk.cu:1839-1861)
@@ -17,15 +17,20 @@
 // on one HIP device.  Conventions follow the reference: messages on stdout,
 // "Error: ..." and exit status 0 on a failed run.
 //
-// Options the reference has that this driver does not implement are
-// rejected with an error message: profiles other than 1 (Voigt), ExoMol /
-// Kurucz / NIST / VALD databases, CIA, bins and output-edges files,
-// "Nnu per bin", sub-Lorentzian wings, removePlinth, transmission and
-// mean opacities.
+// Line files of every database the reference reads: HITRAN/HITEMP/CDSD
+// records (Database 0, 1, 3) and the ExoMol (2), super-line (20), Kurucz
+// (30), NIST (31) and VALD (32) binary files.  Lines stream through the
+// device in chunks (pinned double buffers, device sort by centre).
 //
-// Extension (not in the reference): the optional param.dat key
+// Options the reference has that this driver does not implement are
+// rejected with an error message: profiles other than 1 (Voigt), CIA, bins
+// and output-edges files, "Nnu per bin", sub-Lorentzian wings,
+// removePlinth, transmission and mean opacities.
+//
+// Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
-// Voigt form (0, default) or the FP32-compensated form (1).
+// Voigt form (0, default) or the FP32-compensated form (1); the console
+// flag "-maxlines N" sets the lines per device chunk (default 2^26).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,6 +62,7 @@ struct Param {
     double kmin = 0.0, qalphaL = 1.0, gammaF = 1.0;
     int doMean = 0, units = 0, replaceFiles = 0, profile = 1, doTuning = 1, removePlinth = 0;
     int dev = 0, precision = 0;
+    long long maxlines = 1LL << 26;     // lines per device chunk (console flag -maxlines)
     int nP = 1, nSpecies = 1;
 };
 
@@ -213,6 +219,7 @@ bool read_parameters(Param &p, const char *file, int argc, char **argv) {
         else if (a == "-Mean") p.doMean = std::atoi(v);
         else if (a == "-tuning") p.doTuning = std::atoi(v);
         else if (a == "-prec") p.precision = std::atoi(v);
+        else if (a == "-maxlines") p.maxlines = std::atoll(v);
         else {
             std::printf("Error: Console arguments not valid!\n");
             return false;
@@ -433,103 +440,6 @@ bool read_partition(const Molecule &m, double T, std::vector<double> &Q) {
     return true;
 }
 
-// Host line list of one species (all its files), before sorting.
-struct HostLines {
-    std::vector<double> nu, S, EL, A, delta, vy, n;
-    std::vector<int32_t> iso;
-    void reserve(size_t k) {
-        for (auto *v : {&nu, &S, &EL, &A, &delta, &vy, &n}) v->reserve(k);
-        iso.reserve(k);
-    }
-};
-
-// HITRAN/HITEMP binary records (host.h:605-648): char cid[4], then nu, S,
-// EL, A, delta, gamma_air, gamma_self, n as doubles: 68 bytes, packed.
-bool read_hitran_bin(const std::string &file, long long NL, const Molecule &m, const Param &p,
-                     HostLines &L) {
-    FILE *f = std::fopen(file.c_str(), "rb");
-    if (!f) {
-        std::printf("Error: line list file not found: %s\n", file.c_str());
-        return false;
-    }
-    constexpr size_t kRec = 68;
-    constexpr long long kChunk = 1 << 20;
-    std::vector<unsigned char> buf(kRec * kChunk);
-    for (long long done = 0; done < NL;) {
-        const long long k = std::min(kChunk, NL - done);
-        if (std::fread(buf.data(), kRec, (size_t)k, f) != (size_t)k) {
-            std::printf("Error: line list file too short: %s\n", file.c_str());
-            std::fclose(f);
-            return false;
-        }
-        for (long long r = 0; r < k; ++r) {
-            const unsigned char *q = buf.data() + r * kRec;
-            char cid[5];
-            std::memcpy(cid, q, 4);
-            cid[4] = 0;
-            double v[8];
-            std::memcpy(v, q + 4, sizeof(v));
-            int found = -1;
-            for (size_t j = 0; j < m.iso.size(); ++j)
-                if (std::strcmp(cid, m.iso[j].cid) == 0) found = (int)j;
-            if (found < 0) {
-                std::printf("Error: partition function not found. %lld |%s|%s| \n", done + r, cid,
-                            m.iso[0].cid);
-                std::fclose(f);
-                return false;
-            }
-            L.nu.push_back(v[0]);
-            L.S.push_back(v[1]);
-            L.EL.push_back(v[2]);
-            L.A.push_back(v[3]);
-            L.delta.push_back(v[4]);
-            L.vy.push_back(((1.0 - p.qalphaL) * v[5] + p.qalphaL * v[6]) * p.gammaF);
-            L.n.push_back(v[7]);
-            L.iso.push_back(found);
-        }
-        done += k;
-    }
-    std::fclose(f);
-    return true;
-}
-
-// ExoMol-family binary records (host.h:656-667 reads them, voigt.h:484-578
-// decodes them): per line nd doubles -- nu, S [, E'', A [, Gamma_nat]].
-// Widths are the species' defaults (vy = defaultL gammaF, n = defaultn),
-// there is no pressure shift and one isotopologue.
-bool read_exomol_bin(const std::string &file, long long NL, int nd, const Molecule &m,
-                     const Param &p, HostLines &L) {
-    FILE *f = std::fopen(file.c_str(), "rb");
-    if (!f) {
-        std::printf("Error: line list file not found: %s\n", file.c_str());
-        return false;
-    }
-    constexpr long long kChunk = 1 << 20;
-    std::vector<double> buf((size_t)nd * kChunk);
-    for (long long done = 0; done < NL;) {
-        const long long k = std::min(kChunk, NL - done);
-        if (std::fread(buf.data(), sizeof(double) * nd, (size_t)k, f) != (size_t)k) {
-            std::printf("Error: line list file too short: %s\n", file.c_str());
-            std::fclose(f);
-            return false;
-        }
-        for (long long r = 0; r < k; ++r) {
-            const double *v = buf.data() + r * nd;
-            L.nu.push_back(v[0]);
-            L.S.push_back(v[1]);
-            L.EL.push_back(nd > 2 ? v[2] : 0.0);
-            L.A.push_back(nd > 4 ? v[3] + v[4] : (nd > 2 ? v[3] : 0.0));
-            L.delta.push_back(0.0);
-            L.vy.push_back(m.defaultL * p.gammaF);
-            L.n.push_back(m.defaultn);
-            L.iso.push_back(0);
-        }
-        done += k;
-    }
-    std::fclose(f);
-    return true;
-}
-
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
@@ -563,12 +473,194 @@ double seconds_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// Voigt sums of one species onto K[iP * Nx ...] for every pressure.
+// Pinned host chunk of raw lines: nu, S, EL, A, delta, vy, n (FP64) and iso
+// (int32), `cap` lines each, in one hipHostMalloc allocation so the upload
+// is a DMA that overlaps the device work.
+struct PinnedChunk {
+    void *mem = nullptr;
+    long long cap = 0, n = 0;
+    double *f[7] = {};
+    int32_t *iso = nullptr;
+    double max_abs_delta = 0.0;
+    PinnedChunk() = default;
+    PinnedChunk(const PinnedChunk &) = delete;
+    PinnedChunk &operator=(const PinnedChunk &) = delete;
+    ~PinnedChunk() { if (mem) (void)hipHostFree(mem); }
+    bool alloc(long long c) {
+        cap = c;
+        if (hipHostMalloc(&mem, (size_t)c * (7 * sizeof(double) + sizeof(int32_t)) + 64,
+                          hipHostMallocDefault) != hipSuccess) {
+            mem = nullptr;
+            return false;
+        }
+        for (int k = 0; k < 7; ++k) f[k] = (double *)mem + (size_t)k * c;
+        iso = (int32_t *)((double *)mem + (size_t)7 * c);
+        return true;
+    }
+};
+
+// Sequential reader over the line files [fi0, fi1) of one species.  fill()
+// decodes up to cap lines into a chunk, continuing across file boundaries:
+//   HITRAN-style records (host.h:598-649): char cid[4], then nu, S, EL, A,
+//     delta, gamma_air, gamma_self, n as doubles, 68 bytes packed;
+//     vy = ((1 - qalphaL) gamma_air + qalphaL gamma_self) gammaF;
+//   ExoMol-family records (host.h:656-667, decoded as voigt.h:484-578 does):
+//     nd doubles -- nu, S [, E'', A [, Gamma_nat]]; vy = defaultL gammaF,
+//     n = defaultn, no shift, one isotopologue, A + Gamma_nat as the
+//     natural width.
+class LineStream {
+  public:
+    LineStream(const Molecule &m, const Param &p, int fi0, int fi1)
+        : m_(m), p_(p), fi_(fi0), fi1_(fi1), form_(database_form(m.dataBase)),
+          nd_(form_ == HK_FORM_HITRAN ? 0 : database_doubles(m.dataBase)) {}
+    ~LineStream() { if (f_) std::fclose(f_); }
+
+    bool fill(PinnedChunk &c) {
+        c.n = 0;
+        c.max_abs_delta = 0.0;
+        while (c.n < c.cap) {
+            if (left_ == 0) {
+                if (f_) { std::fclose(f_); f_ = nullptr; ++fi_; }
+                if (fi_ >= fi1_) break;
+                if (!open_file()) return false;
+                continue;
+            }
+            const long long k = std::min({left_, c.cap - c.n, kStage});
+            if (!(form_ == HK_FORM_HITRAN ? decode_hitran(c, k) : decode_exomol(c, k))) return false;
+            left_ -= k;
+            c.n += k;
+        }
+        return true;
+    }
+
+  private:
+    static constexpr long long kStage = 1 << 16;   // records per fread
+    static constexpr size_t kRec = 68;
+
+    bool open_file() {
+        name_ = m_.dataFilename[fi_] + "bin";
+        std::printf("Reading Line file %d of %d: %s\n", fi_, fi1_ - 1, name_.c_str());
+        std::printf("Number of lines: %lld\n", m_.NL[fi_]);
+        f_ = std::fopen(name_.c_str(), "rb");
+        if (!f_) {
+            std::printf("Error: line list file not found: %s\n", name_.c_str());
+            return false;
+        }
+        left_ = m_.NL[fi_];
+        if (left_ == 0) { std::fclose(f_); f_ = nullptr; ++fi_; }
+        return true;
+    }
+
+    bool read(size_t rec, long long k) {
+        buf_.resize(rec * (size_t)k);
+        if (std::fread(buf_.data(), rec, (size_t)k, f_) != (size_t)k) {
+            std::printf("Error: line list file too short: %s\n", name_.c_str());
+            return false;
+        }
+        return true;
+    }
+
+    bool decode_hitran(PinnedChunk &c, long long k) {
+        if (!read(kRec, k)) return false;
+        const double q = p_.qalphaL, g = p_.gammaF;
+        for (long long r = 0; r < k; ++r) {
+            const unsigned char *rec = buf_.data() + r * kRec;
+            char cid[5];
+            std::memcpy(cid, rec, 4);
+            cid[4] = 0;
+            double v[8];
+            std::memcpy(v, rec + 4, sizeof(v));
+            int found = -1;
+            for (size_t j = 0; j < m_.iso.size(); ++j)
+                if (std::strcmp(cid, m_.iso[j].cid) == 0) found = (int)j;
+            if (found < 0) {
+                std::printf("Error: partition function not found. %lld |%s|%s| \n",
+                            m_.NL[fi_] - left_ + r, cid, m_.iso[0].cid);
+                return false;
+            }
+            const long long i = c.n + r;
+            c.f[0][i] = v[0];
+            c.f[1][i] = v[1];
+            c.f[2][i] = v[2];
+            c.f[3][i] = v[3];
+            c.f[4][i] = v[4];
+            c.f[5][i] = ((1.0 - q) * v[5] + q * v[6]) * g;
+            c.f[6][i] = v[7];
+            c.iso[i] = found;
+            c.max_abs_delta = std::max(c.max_abs_delta, std::fabs(v[4]));
+        }
+        return true;
+    }
+
+    bool decode_exomol(PinnedChunk &c, long long k) {
+        if (!read(sizeof(double) * nd_, k)) return false;
+        const double vy = m_.defaultL * p_.gammaF, ne = m_.defaultn;
+        for (long long r = 0; r < k; ++r) {
+            double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            std::memcpy(v, buf_.data() + r * sizeof(double) * nd_, sizeof(double) * nd_);
+            const long long i = c.n + r;
+            c.f[0][i] = v[0];
+            c.f[1][i] = v[1];
+            c.f[2][i] = v[2];
+            c.f[3][i] = v[3] + v[4];
+            c.f[4][i] = 0.0;
+            c.f[5][i] = vy;
+            c.f[6][i] = ne;
+            c.iso[i] = 0;
+        }
+        return true;
+    }
+
+    const Molecule &m_;
+    const Param &p_;
+    int fi_, fi1_, form_, nd_;
+    FILE *f_ = nullptr;
+    long long left_ = 0;
+    std::string name_;
+    std::vector<unsigned char> buf_;
+};
+
+struct Timing {
+    double read = 0.0, lines = 0.0, voigt = 0.0;
+};
+
+// K[iP * Nx ...] = kmin for every pressure: an empty line list summed with
+// accumulate = 0, so that every species and chunk afterwards accumulates.
+bool init_K(const Param &p, const hk_grid &grid, size_t nP, double *K_d, hipStream_t s) {
+    hk_lines L;
+    std::memset(&L, 0, sizeof(L));
+    L.niso = 1;
+    L.iso_mass[0] = 1.0;
+    DevBuf dst, dws;
+    hk_line_state st;
+    const size_t stb = hk_line_state_bytes(0);
+    if (!dst.alloc(stb)) return fail("Error: device allocation failed");
+    HKCHK(hk_line_state_init(&st, 0, dst.p, stb));
+    hk_tune tune = {0, 0};
+    const size_t wsb = hk_voigt_workspace_bytes(&grid, &tune);
+    if (!dws.alloc(wsb)) return fail("Error: device allocation of the Voigt workspace failed");
+    hk_tp tp;
+    std::memset(&tp, 0, sizeof(tp));
+    tp.T = 296.0;
+    for (int j = 0; j < HK_MAX_ISO; ++j) tp.Q[j] = 1.0;
+    hk_cut c = {0, 1.0};
+    HKCHK(hk_line_params(&L, &tp, &c, &st, s));
+    for (size_t iP = 0; iP < nP; ++iP)
+        HKCHK(hk_voigt_accumulate(&st, &grid, p.kmin, 0, HK_VOIGT_FP64, &tune,
+                                  K_d + iP * grid.nx, dws.p, wsb, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return true;
+}
+
+// Voigt sums of one species added onto K[iP * Nx ...] for every pressure.
+// The species' lines stream through in chunks of at most maxlines (the
+// reference's def_maxlines parts, heliosk.cu:988-1051): each chunk is read
+// into one of two pinned host buffers, uploaded, sorted by centre on the
+// device (hk_lines_sort) and summed at every pressure; the host reads the
+// next chunk into the other buffer while the device works on this one.
 bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q, double Sscale,
                  double meanMass, const hk_grid &grid, const std::vector<double> &Plist,
-                 bool first, double *K_d, hipStream_t s, double &t_read, double &t_lines,
-                 double &t_voigt) {
-    auto t0 = std::chrono::steady_clock::now();
+                 double *K_d, hipStream_t s, long long maxlines, Timing &tm) {
     // file range (heliosk.cu:939-957)
     int fi0 = m.nFiles, fi1 = 0;
     const double cut = p.cut == 0.0 ? 1.0e30 : p.cut;
@@ -586,105 +678,141 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
         fi1 = m.nFiles;
     }
     std::printf("File range %d to %d\n", fi0, fi1 - 1);
-    HostLines H;
     long long total = 0;
     for (int fi = fi0; fi < fi1; ++fi) total += m.NL[fi];
-    H.reserve((size_t)std::max(0LL, total));
-    for (int fi = fi0; fi < fi1; ++fi) {
-        const std::string fn = m.dataFilename[fi] + "bin";
-        std::printf("Reading Line file %d of %d: %s\n", fi, fi1 - 1, fn.c_str());
-        std::printf("Number of lines: %lld\n", m.NL[fi]);
-        const int form = database_form(m.dataBase);
-        const bool ok = form == HK_FORM_HITRAN
-                            ? read_hitran_bin(fn, m.NL[fi], m, p, H)
-                            : read_exomol_bin(fn, m.NL[fi], database_doubles(m.dataBase), m, p, H);
-        if (!ok) return false;
-    }
-    const int64_t nl = (int64_t)H.nu.size();
-    // lines sorted by centre (stable: ties keep file order)
-    std::vector<int64_t> order(nl);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return H.nu[a] < H.nu[b]; });
-    auto gather = [&](std::vector<double> &v) {
-        std::vector<double> o(nl);
-        for (int64_t i = 0; i < nl; ++i) o[i] = v[order[i]];
-        v.swap(o);
-    };
-    for (auto *v : {&H.nu, &H.S, &H.EL, &H.A, &H.delta, &H.vy, &H.n}) gather(*v);
-    {
-        std::vector<int32_t> o(nl);
-        for (int64_t i = 0; i < nl; ++i) o[i] = H.iso[order[i]];
-        H.iso.swap(o);
-    }
-    t_read += seconds_since(t0);
+    if (total <= 0) return true;
+    const long long cap = std::min(maxlines, total);
+    const bool two = total > cap;
 
-    // device line list: 7 FP64 fields + iso
-    DevBuf dl, dst, dws;
-    const size_t n8 = (size_t)nl * sizeof(double);
-    if (!dl.alloc(7 * n8 + (size_t)nl * sizeof(int32_t) + 256)) {
-        std::printf("Error: device allocation of %lld lines failed\n", (long long)nl);
-        return false;
-    }
-    double *b = dl.as<double>();
-    std::vector<double> *fields[7] = {&H.nu, &H.S, &H.EL, &H.A, &H.delta, &H.vy, &H.n};
-    for (int k = 0; k < 7; ++k)
-        if (nl) HIPCHK(hipMemcpy(b + k * nl, fields[k]->data(), n8, hipMemcpyHostToDevice));
-    int32_t *iso_d = (int32_t *)(b + 7 * nl);
-    if (nl) HIPCHK(hipMemcpy(iso_d, H.iso.data(), (size_t)nl * sizeof(int32_t), hipMemcpyHostToDevice));
+    PinnedChunk pc[2];
+    if (!pc[0].alloc(cap) || (two && !pc[1].alloc(cap)))
+        return fail("Error: pinned host allocation of %lld lines failed", cap);
+    // device: raw chunk, sorted chunk, line state, sort and Voigt workspaces
+    const size_t n8 = (size_t)cap * sizeof(double);
+    const size_t listb = 7 * n8 + (size_t)cap * sizeof(int32_t) + 256;
+    DevBuf draw, dsorted, dst, dsws, dws;
+    if (!draw.alloc(listb) || !dsorted.alloc(listb))
+        return fail("Error: device allocation of %lld lines failed", cap);
+    const size_t stb = hk_line_state_bytes(cap);
+    const size_t sortb = hk_lines_sort_workspace_bytes(cap);
+    if (sortb == 0) return fail("Error: %s", hk_last_error());
+    hk_tune tune = {0, 0};
+    const size_t wsb = hk_voigt_workspace_bytes(&grid, &tune);
+    if (!dst.alloc(stb) || !dsws.alloc(sortb) || !dws.alloc(wsb))
+        return fail("Error: device allocation of the line state and workspaces failed");
 
-    hk_lines L;
-    std::memset(&L, 0, sizeof(L));
-    L.nlines = nl;
-    L.nu = b; L.S = b + nl; L.EL = b + 2 * nl; L.A = b + 3 * nl; L.delta = b + 4 * nl;
-    L.vy = b + 5 * nl; L.n = b + 6 * nl; L.iso = iso_d;
-    L.niso = database_form(m.dataBase) == HK_FORM_HITRAN ? (int)m.iso.size() : 1;
-    L.form = database_form(m.dataBase);
-    double mdelta = 0.0;
-    for (double d : H.delta) mdelta = std::max(mdelta, std::fabs(d));
-    L.max_abs_delta = mdelta;
-    for (int j = 0; j < L.niso; ++j) {
-        // readFile's strength scaling (host.h:628-644)
+    hk_lines base;
+    std::memset(&base, 0, sizeof(base));
+    base.form = database_form(m.dataBase);
+    base.niso = base.form == HK_FORM_HITRAN ? (int)m.iso.size() : 1;
+    for (int j = 0; j < base.niso; ++j) {
+        // readFile's strength scaling (host.h:628-644; heliosk.cu:1062-1070)
         double Ab = m.iso[j].Ab, ss = Sscale;
         if (p.units == 0) {
             Ab *= m.iso[j].m / meanMass;
             ss *= m.iso[j].m / meanMass;
         }
-        L.iso_mass[j] = m.iso[j].m / kNA;
-        L.iso_ab[j] = Ab;
-        L.iso_sscale[j] = ss;
+        base.iso_mass[j] = m.iso[j].m / kNA;
+        base.iso_ab[j] = Ab;
+        base.iso_sscale[j] = ss;
     }
-    hk_line_state st;
-    const size_t stb = hk_line_state_bytes(nl);
-    if (!dst.alloc(stb)) {
-        std::printf("Error: device allocation of the line state failed\n");
-        return false;
-    }
-    HKCHK(hk_line_state_init(&st, nl, dst.p, stb));
-    hk_tune tune = {0, 0};
-    const size_t wsb = hk_voigt_workspace_bytes(&grid, &tune);
-    if (!dws.alloc(wsb)) {
-        std::printf("Error: device allocation of the Voigt workspace failed\n");
-        return false;
-    }
+    auto list_at = [&](void *mem, long long n) {
+        hk_lines L = base;
+        double *b = (double *)mem;
+        L.nlines = n;
+        L.nu = b; L.S = b + cap; L.EL = b + 2 * cap; L.A = b + 3 * cap; L.delta = b + 4 * cap;
+        L.vy = b + 5 * cap; L.n = b + 6 * cap; L.iso = (int32_t *)(b + 7 * cap);
+        return L;
+    };
     hk_cut c = {p.cutMode, p.cut};
     const int mode = p.precision == 1 ? HK_VOIGT_FP32C : HK_VOIGT_FP64;
-    for (size_t iP = 0; iP < Plist.size(); ++iP) {
+
+    // timing events: [sort start, params start, voigt start, voigt end] per step
+    std::vector<hipEvent_t> ev;
+    auto event = [&](hipEvent_t &e) {
+        if (hipEventCreate(&e) != hipSuccess) return false;
+        ev.push_back(e);
+        return hipEventRecord(e, s) == hipSuccess;
+    };
+    struct Mark { size_t sort0, sort1; std::vector<size_t> par, voi, end; };
+    std::vector<Mark> marks;
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    bool ok = hipEventCreateWithFlags(&copied[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&copied[1], hipEventDisableTiming) == hipSuccess;
+
+    LineStream rs(m, p, fi0, fi1);
+    auto t0 = std::chrono::steady_clock::now();
+    ok = ok && rs.fill(pc[0]);
+    tm.read += seconds_since(t0);
+    int b = 0;
+    long long part = 0;
+    while (ok && pc[b].n > 0) {
+        const long long n = pc[b].n;
+        std::printf("Line chunk %lld: %lld lines\n", part, n);
+        hk_lines raw = list_at(draw.p, n), srt = list_at(dsorted.p, n);
+        raw.max_abs_delta = pc[b].max_abs_delta;
+        for (int k = 0; k < 7 && ok; ++k)
+            ok = hipMemcpyAsync((double *)draw.p + (size_t)k * cap, pc[b].f[k],
+                                (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync((double *)draw.p + (size_t)7 * cap, pc[b].iso,
+                                  (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s) == hipSuccess;
+        ok = ok && hipEventRecord(copied[b], s) == hipSuccess;
+        if (!ok) break;
+        used[b] = true;
+        Mark mk;
+        hipEvent_t e;
+        mk.sort0 = ev.size();
+        if (!event(e)) { ok = false; break; }
+        HKCHK(hk_lines_sort(&raw, &srt, dsws.p, sortb, s));
+        hk_line_state st;
+        HKCHK(hk_line_state_init(&st, n, dst.p, stb));
+        for (size_t iP = 0; iP < Plist.size() && ok; ++iP) {
+            hk_tp tp;
+            std::memset(&tp, 0, sizeof(tp));
+            tp.T = p.T;
+            tp.P = Plist[iP];
+            for (int j = 0; j < HK_MAX_ISO; ++j) tp.Q[j] = j < (int)Q.size() ? Q[j] : 1.0;
+            mk.par.push_back(ev.size());
+            if (!event(e)) { ok = false; break; }
+            HKCHK(hk_line_params(&srt, &tp, &c, &st, s));
+            mk.voi.push_back(ev.size());
+            if (!event(e)) { ok = false; break; }
+            HKCHK(hk_voigt_accumulate(&st, &grid, p.kmin, 1, mode, &tune, K_d + iP * grid.nx,
+                                      dws.p, wsb, s));
+            mk.end.push_back(ev.size());
+            if (!event(e)) { ok = false; break; }
+        }
+        marks.push_back(mk);
+        if (!ok) break;
+        // the next chunk goes into the other buffer once its last upload is done
+        const int nb = two ? 1 - b : b;
+        if (used[nb]) ok = hipEventSynchronize(copied[nb]) == hipSuccess;
         auto t1 = std::chrono::steady_clock::now();
-        hk_tp tp;
-        std::memset(&tp, 0, sizeof(tp));
-        tp.T = p.T;
-        tp.P = Plist[iP];
-        for (int j = 0; j < HK_MAX_ISO; ++j) tp.Q[j] = j < (int)Q.size() ? Q[j] : 1.0;
-        HKCHK(hk_line_params(&L, &tp, &c, &st, s));
-        HIPCHK(hipStreamSynchronize(s));
-        t_lines += seconds_since(t1);
-        t1 = std::chrono::steady_clock::now();
-        HKCHK(hk_voigt_accumulate(&st, &grid, p.kmin, first ? 0 : 1, mode, &tune,
-                                  K_d + iP * grid.nx, dws.p, wsb, s));
-        HIPCHK(hipStreamSynchronize(s));
-        t_voigt += seconds_since(t1);
+        ok = ok && rs.fill(pc[nb]);
+        tm.read += seconds_since(t1);
+        b = nb;
+        ++part;
     }
+    const bool synced = hipStreamSynchronize(s) == hipSuccess;
+    if (ok && synced) {
+        for (const Mark &mk : marks) {
+            float ms = 0.0f;
+            if (mk.par.empty()) continue;
+            if (hipEventElapsedTime(&ms, ev[mk.sort0], ev[mk.par[0]]) == hipSuccess)
+                tm.lines += 1e-3 * ms;
+            for (size_t k = 0; k < mk.par.size() && k < mk.end.size(); ++k) {
+                if (hipEventElapsedTime(&ms, ev[mk.par[k]], ev[mk.voi[k]]) == hipSuccess)
+                    tm.lines += 1e-3 * ms;
+                if (hipEventElapsedTime(&ms, ev[mk.voi[k]], ev[mk.end[k]]) == hipSuccess)
+                    tm.voigt += 1e-3 * ms;
+            }
+        }
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : copied) if (e) (void)hipEventDestroy(e);
+    if (!synced) return fail("Error: HIP stream synchronisation failed");
+    if (!ok) return fail("Error: line chunk %lld failed", part);
     return true;
 }
 
@@ -748,6 +876,8 @@ bool run(int argc, char **argv) {
     if (p.doMean > 0) return fail("Error: mean opacities are not supported");
     if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
     if (p.nbins <= 0) return fail("Error: nbins must be > 0");
+    if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
+        return fail("Error: maxlines must lie in 1 .. 2147483647");
     if (!(p.dnu > 0.0)) return fail("Error: dnu must be > 0");
 
     std::vector<double> Plist(1, p.P);
@@ -827,24 +957,25 @@ bool run(int argc, char **argv) {
     std::vector<double> K_h(Nx), x_h(Nx);
     for (int64_t j = 0; j < Nx; ++j) x_h[j] = std::fma((double)j, p.dnu, p.numin);
 
-    double t_read = 0.0, t_lines = 0.0, t_voigt = 0.0;
+    Timing tm;
     auto t_all = std::chrono::steady_clock::now();
     if (p.doStoreFullK >= 0) {
+        if (!init_K(p, grid, Plist.size(), K_d, s)) { std::fclose(info); return false; }
         for (size_t k = 0; k < mols.size(); ++k) {
             std::vector<double> Q;
             if (!read_partition(mols[k], p.T, Q)) { std::fclose(info); return false; }
             const double Sscale = p.nSpecies > 1 ? spAb[k] : 1.0;
             std::printf("mean mass %g, Sscale %g\n", meanMass, Sscale);
-            if (!add_species(p, mols[k], Q, Sscale, meanMass, grid, Plist, k == 0, K_d, s, t_read,
-                             t_lines, t_voigt)) {
+            if (!add_species(p, mols[k], Q, Sscale, meanMass, grid, Plist, K_d, s, p.maxlines,
+                             tm)) {
                 std::fclose(info);
                 return false;
             }
         }
     }
-    std::printf("Time for input:        %g seconds\n", t_read);
-    std::printf("Time for Lines:        %g seconds\n", t_lines);
-    std::printf("Time for K(x):         %g seconds\n", t_voigt);
+    std::printf("Time for input:        %g seconds\n", tm.read);
+    std::printf("Time for Lines:        %g seconds\n", tm.lines);
+    std::printf("Time for K(x):         %g seconds\n", tm.voigt);
 
     // full cross sections
     if (p.doStoreFullK == 1) {
@@ -997,9 +1128,9 @@ bool run(int argc, char **argv) {
     const double tk = seconds_since(t_kt);
     std::printf("Time for Resampling:   %g seconds\n", tk);
     const double tt = seconds_since(t_all);
-    std::fprintf(info, "\nTime for input:        %g seconds\n", t_read);
-    std::fprintf(info, "Time for Lines:        %g seconds\n", t_lines);
-    std::fprintf(info, "Time for K(x):         %g seconds\n", t_voigt);
+    std::fprintf(info, "\nTime for input:        %g seconds\n", tm.read);
+    std::fprintf(info, "Time for Lines:        %g seconds\n", tm.lines);
+    std::fprintf(info, "Time for K(x):         %g seconds\n", tm.voigt);
     std::fprintf(info, "Time for Resampling:   %g seconds\n", tk);
     std::fprintf(info, "Total time:            %g seconds\n", tt);
     std::fclose(info);

@@ -18,7 +18,8 @@ from ._lib import (HK_FORM_EXOMOL, HK_FORM_HITRAN, HK_FORM_SUPERLINE, HK_VOIGT_F
 
 NA = 6.0221412927e23          # define.h:42
 
-__all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
+__all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
+           "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
            "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
@@ -100,6 +101,49 @@ class Species:
             L.iso_sscale[k] = self.iso_sscale[k]
         L.max_abs_delta = self.max_abs_delta
         return L
+
+
+def sort_lines(nu, S, EL, vy, n, A=None, delta=None, iso=None, iso_mass=(18.0 / NA,),
+               iso_ab=(1.0,), iso_sscale=(1.0,), name="species", form=HK_FORM_HITRAN,
+               stream=None):
+    """Species from an unsorted line list already on the device, ordered by
+    the library's stable radix sort (hk_lines_sort; the thrust sort and
+    gathers of heliosk.cu:1147-1171).  Per-line arguments are device
+    tensors (float64; iso int32) or array-likes, which are uploaded."""
+    dev = _dev()
+
+    def on_dev(a, dtype=torch.float64):
+        t = a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a))
+        return t.to(device=dev, dtype=dtype).contiguous()
+
+    raw = {k: on_dev(v) for k, v in (("nu", nu), ("S", S), ("EL", EL), ("vy", vy), ("n", n))}
+    nl = raw["nu"].numel()
+    raw["A"] = on_dev(A) if A is not None else torch.zeros(nl, dtype=torch.float64, device=dev)
+    raw["delta"] = (on_dev(delta) if delta is not None
+                    else torch.zeros(nl, dtype=torch.float64, device=dev))
+    raw_iso = on_dev(iso, torch.int32) if iso is not None else None
+    out = {k: torch.empty_like(v) for k, v in raw.items()}
+    out_iso = torch.empty_like(raw_iso) if raw_iso is not None else None
+    sp = object.__new__(Species)
+    sp.name = name; sp.form = int(form); sp.order = None
+    sp.iso_mass = list(iso_mass); sp.iso_ab = list(iso_ab); sp.iso_sscale = list(iso_sscale)
+    sp.nlines = nl
+    sp.max_abs_delta = float(raw["delta"].abs().max()) if nl else 0.0
+    for k, v in raw.items():
+        setattr(sp, k, v)
+    sp.iso = raw_iso
+    Lin = sp.lines()
+    for k, v in out.items():
+        setattr(sp, k, v)
+    sp.iso = out_iso
+    Lout = sp.lines()
+    nbytes = lib().hk_lines_sort_workspace_bytes(nl)
+    if nbytes == 0:
+        raise _lib.HKError("hk_lines_sort_workspace_bytes(%d) failed" % nl)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    check(lib().hk_lines_sort(ctypes.byref(Lin), ctypes.byref(Lout), _ptr(ws), nbytes,
+                              _stream(stream)))
+    return sp
 
 
 class State:

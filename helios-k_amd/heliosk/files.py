@@ -173,7 +173,17 @@ def species_files(dirpath, name, lines, isotopologues, T_table, Q_tables, limits
         isos.append((cid, ab, q, g, m, qf))
     nl = len(lines["nu"])
     lim = limits if limits is not None else [0, int(np.ceil(np.max(lines["nu"]))) + 1 if nl else 1]
-    write_species_param(os.path.join(dirpath, name + ".param"), name, isos, [nl], lim)
-    write_hitran_bin(os.path.join(dirpath, name + ".bin"), lines["cid"], lines["nu"], lines["S"],
-                     lines["EL"], lines["A"], lines["delta"], lines["gair"], lines["gself"],
-                     lines["n"])
+    nfiles = len(lim) - 1
+    # one file per [lim[k], lim[k+1]) range, lines in their given order
+    fidx = np.clip(np.searchsorted(np.asarray(lim[1:-1], dtype=float), lines["nu"], side="right"),
+                   0, nfiles - 1) if nfiles > 1 else np.zeros(nl, dtype=int)
+    counts = []
+    cid = np.asarray(lines["cid"]) if np.ndim(lines["cid"]) else np.array([lines["cid"]] * nl)
+    for k in range(nfiles):
+        sel = np.nonzero(fidx == k)[0]
+        counts.append(sel.size)
+        stem = species_param_name(name, 0, nfiles, lim[k], lim[k + 1])
+        write_hitran_bin(os.path.join(dirpath, stem + "bin"), list(cid[sel]),
+                         *[np.asarray(lines[f])[sel] for f in ("nu", "S", "EL", "A", "delta", "gair",
+                                                                "gself", "n")])
+    write_species_param(os.path.join(dirpath, name + ".param"), name, isos, counts, lim)

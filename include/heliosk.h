@@ -10,6 +10,8 @@
  * calling thread's last error.
  *
  * Reference seams replaced (exoclime/HELIOS-K 2.08):
+ *   hk_lines_sort        thrust::sort_by_key + Sort_kernel/Copy_kernel
+ *                        (heliosk.cu:1147-1171, voigt.h:843-866)
  *   hk_line_params       S2_kernel (voigt.h:331-363) + the per-line part of
  *                        readFile (host.h:641-644) + the cut of S3_kernel
  *                        (voigt.h:436-482)
@@ -134,6 +136,20 @@ const char *hk_version(void);
 size_t hk_line_state_bytes(int64_t nlines);
 /* Lay out an hk_line_state inside caller memory of hk_line_state_bytes. */
 int hk_line_state_init(hk_line_state *st, int64_t nlines, void *mem, size_t bytes);
+
+/* Order a raw line list by centre: the device sort of main
+ * (thrust::sort_by_key on nu, then the Sort_kernel / Copy_kernel gathers of
+ * every field, heliosk.cu:1147-1171, voigt.h:843-866).  Stable: lines with
+ * equal nu keep their input order.  `in` may be in any order; the arrays
+ * `out` points to (nu, S, EL, A, delta, vy, n and, when in->iso is not NULL,
+ * iso) receive the sorted fields and must be distinct from in's, each with
+ * room for in->nlines entries.  out's scalar fields (nlines, niso, form, the
+ * isotopologue tables, max_abs_delta) are set from in.  At most 2^31 - 1
+ * lines per call; the workspace holds two permutations and the radix-sort
+ * scratch. */
+size_t hk_lines_sort_workspace_bytes(int64_t nlines);
+int hk_lines_sort(const hk_lines *in, hk_lines *out, void *workspace, size_t bytes,
+                  hk_stream_t stream);
 
 /* Line parameters at one (T,P) -- S2_kernel semantics, FP64. */
 int hk_line_params(const hk_lines *lines, const hk_tp *tp, const hk_cut *cut,

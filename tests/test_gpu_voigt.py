@@ -282,3 +282,24 @@ def test_config1_precision_at_scale(E):
     print("c1: FP64 vs oracle %.3e, FP32C vs oracle %.3e, FP32C vs FP64 (full grid) %.3e"
           % (e64, e32, e3264))
     assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
+
+
+@pytest.mark.parametrize("n", [1, 1000, 300001])
+def test_device_line_sort_is_stable(E, n):
+    """hk_lines_sort against numpy's stable argsort: the same permutation,
+    bit for bit, with many equal centres (ties keep input order)."""
+    import torch
+    rng = np.random.default_rng(n)
+    nu = np.round(rng.uniform(0.0, 3000.0, n), 1)        # ~30000 distinct values: many ties
+    f = {k: rng.uniform(0.0, 1.0, n) for k in ("S", "EL", "A", "delta", "vy", "n")}
+    iso = rng.integers(0, 4, n).astype(np.int32)
+    sp = E.sort_lines(nu, f["S"], f["EL"], f["vy"], f["n"], A=f["A"], delta=f["delta"], iso=iso,
+                      iso_mass=(1.0, 2.0, 3.0, 4.0), iso_ab=(1.0,) * 4, iso_sscale=(1.0,) * 4,
+                      form=E.HK_FORM_EXOMOL)
+    torch.cuda.synchronize()
+    o = np.argsort(nu, kind="stable")
+    np.testing.assert_array_equal(sp.nu.cpu().numpy(), nu[o])
+    for k, v in f.items():
+        np.testing.assert_array_equal(getattr(sp, k).cpu().numpy(), v[o], err_msg=k)
+    np.testing.assert_array_equal(sp.iso.cpu().numpy(), iso[o])
+    assert sp.form == E.HK_FORM_EXOMOL and sp.nlines == n
