@@ -22,10 +22,14 @@
 // (30), NIST (31) and VALD (32) binary files.  Lines stream through the
 // device in chunks (pinned double buffers, device sort by centre).
 //
+// Outputs beyond K and the k-tables: Planck / Rosseland means
+// (Out_<name>_mean.dat, doMean), transmission functions (Out_<name>_tr.dat /
+// _tr%04d.dat, doTransmission 1 / 2) and K(y) averaged between output edges
+// (OutputEdgesFile).
+//
 // Options the reference has that this driver does not implement are
 // rejected with an error message: profiles other than 1 (Voigt), CIA, bins
-// and output-edges files, "Nnu per bin", sub-Lorentzian wings,
-// removePlinth, transmission and mean opacities.
+// files, "Nnu per bin", sub-Lorentzian wings and removePlinth.
 //
 // Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
@@ -858,7 +862,46 @@ bool write_info(FILE *f, const Param &p, int64_t Nx, const char *device) {
     std::fprintf(f, "doTuning = %d\n", p.doTuning);
     std::fprintf(f, "precision = %d (%s)\n", p.precision,
                  p.precision == 1 ? "FP32 profile, compensated sums" : "FP64");
+    if (p.edges != "-") std::fprintf(f, "use output edges: %s\n", p.edges.c_str());
     return true;
+}
+
+// K(y) of one bin in the Out_<name>_bin layout (heliosk.cu:2245-2290):
+// "y K" rows, or -- with an output-edges file -- the mean of K over each
+// interval between consecutive edges, at the interval's midpoint.  The
+// running sum includes point j when edge e falls in [y_j, y_{j+1}], then
+// restarts; the first edge only starts the first interval.
+void write_bin_K(FILE *o, const Param &p, const double *K, int64_t Nxb, double unitScale,
+                 const std::vector<double> &edges, double P) {
+    const int ne = (int)edges.size();
+    int iedge = 0, nedge = 0;
+    double sedge = 0.0;
+    for (int64_t j = 0; j < Nxb; ++j) {
+        const double y = j / (double)(Nxb - 1);
+        const double y1 = (j + 1) / (double)(Nxb - 1);
+        const double k = K[j] * unitScale;
+        if (ne == 0) {
+            if (p.nP == 1) std::fprintf(o, "%g %.20g\n", y, k);
+            else std::fprintf(o, "%g %.20g %g %g %lld\n", y, k, p.T, P, (long long)j);
+            continue;
+        }
+        ++nedge;
+        sedge += k;
+        if (iedge < ne) {
+            const double edge = edges[iedge];
+            if (y <= edge && edge <= y1) {
+                if (iedge > 0) {
+                    const double mid = 0.5 * (edge + edges[iedge - 1]);
+                    if (p.nP == 1) std::fprintf(o, "%g %.20g\n", mid, sedge / (double)nedge);
+                    else std::fprintf(o, "%g %.20g %g %g %d\n", mid, sedge / (double)nedge, p.T, P,
+                                      iedge - 1);
+                }
+                ++iedge;
+                nedge = 0;
+                sedge = 0.0;
+            }
+        }
+    }
 }
 
 bool run(int argc, char **argv) {
@@ -870,15 +913,24 @@ bool run(int argc, char **argv) {
     if (p.subLorentzian != "-") return fail("Error: sub-Lorentzian files are not supported");
     if (p.ciaSystem != "-") return fail("Error: CIA systems are not supported");
     if (p.bins != "-") return fail("Error: bins files are not supported");
-    if (p.edges != "-") return fail("Error: output-edges files are not supported");
     if (p.Nxb != 0) return fail("Error: 'Nnu per bin' is not supported");
-    if (p.doTransmission > 0) return fail("Error: transmission functions are not supported");
-    if (p.doMean > 0) return fail("Error: mean opacities are not supported");
     if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
     if (p.nbins <= 0) return fail("Error: nbins must be > 0");
     if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
         return fail("Error: maxlines must lie in 1 .. 2147483647");
     if (!(p.dnu > 0.0)) return fail("Error: dnu must be > 0");
+
+    // output edges (host.h:412-431, 547-564)
+    std::vector<double> edges;
+    if (p.edges != "-") {
+        std::printf("Use output edges in file %s\n", p.edges.c_str());
+        int ne = 0;
+        if (!count_values(p.edges, ne, "output edges")) return false;
+        if (!read_values(p.edges, edges)) return fail("Error: output edges file not found: %s", p.edges.c_str());
+        for (size_t i = 0; i < edges.size(); ++i)
+            if (!(edges[i] > (i ? edges[i - 1] : -100000.0)))
+                return fail("Error; output edges not monotonic growing");
+    }
 
     std::vector<double> Plist(1, p.P);
     if (p.PFilename != "-") {
@@ -1034,20 +1086,65 @@ bool run(int argc, char **argv) {
         std::fclose(o);
     }
 
-    // k-tables and per-bin K
+    // Planck and Rosseland means of every pressure (heliosk.cu:1953-2012)
+    auto t_mean = std::chrono::steady_clock::now();
+    if (p.doMean > 0) {
+        DevBuf mws, msum;
+        const size_t mwb = hk_means_workspace_bytes();
+        if (!mws.alloc(mwb) || !msum.alloc(4 * sizeof(double)))
+            return fail("Error: allocation for the means failed");
+        const std::string fn = "Out_" + p.name + "_mean.dat";
+        FILE *o = std::fopen(fn.c_str(), fmode);
+        if (!o) return fail("Error: cannot open %s", fn.c_str());
+        const double kB = 1.3806489e-16, h = 6.62606957e-27, c = 2.99792458e10;  // define.h:39-41
+        const double sigma = 2.0 * kB * kB * kB * kB / (h * h * h * c * c * 15.0) * M_PI * M_PI *
+                             M_PI * M_PI * M_PI;
+        const double T = p.T;
+        const double integral1 = sigma * T * T * T * T / M_PI;
+        const double integral2 = M_PI / (4.0 * sigma * T * T * T);
+        for (int iP = 0; iP < p.nP; ++iP) {
+            double m[4];
+            HKCHK(hk_means(K_d + (size_t)iP * Nx, &grid, T, msum.as<double>(), mws.p, mwb, s));
+            HIPCHK(hipMemcpyAsync(m, msum.p, sizeof(m), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (p.nP == 1) {
+                std::fprintf(o, "%.20g\n", m[0] / m[2]);
+                std::fprintf(o, "%.20g\n", m[3] / m[1]);
+                std::fprintf(o, "%.20g\n", m[2] * p.dnu);
+                std::fprintf(o, "%.20g\n", integral1);
+                std::fprintf(o, "%.20g\n", m[3] * p.dnu);
+                std::fprintf(o, "%.20g\n", 1.0 / integral2);
+            } else {
+                // the reference writes the two integrals without the dnu factor here
+                const double P = Plist[iP];
+                std::fprintf(o, "%.20g %.20g %.20g\n", m[0] / m[2], T, P);
+                std::fprintf(o, "%.20g %.20g %.20g\n", m[3] / m[1], T, P);
+                std::fprintf(o, "%.20g %.20g %.20g\n", m[2], T, P);
+                std::fprintf(o, "%.20g %.20g %.20g\n", integral1, T, P);
+                std::fprintf(o, "%.20g %.20g %.20g\n", m[3], T, P);
+                std::fprintf(o, "%.20g %.20g %.20g\n", 1.0 / integral2, T, P);
+            }
+        }
+        std::fclose(o);
+    }
+    const double tmean = seconds_since(t_mean);
+    std::printf("Time for mean K(x):    %g seconds\n", tmean);
+
+    // k-tables, per-bin K and the transmission input
     auto t_kt = std::chrono::steady_clock::now();
-    const bool need_sorted = p.doStoreK > 0 && p.doResampling == 0;
+    const bool need_sorted = (p.doStoreK > 0 || p.doTransmission > 0) && p.doResampling == 0;
+    const bool need_recon = (p.doStoreK > 0 || p.doTransmission > 0) && p.doResampling > 0;
     DevBuf ws, coeff, nxmin, Ks, Ky, scr;
     std::vector<double> coeff_h;
     std::vector<int32_t> nxmin_h;
-    if (p.doResampling > 0 || p.doStoreK > 0) {
+    if (p.doResampling > 0 || p.doStoreK > 0 || p.doTransmission > 0) {
         const int nC = std::max(p.nC, 1);
         const size_t wsb = hk_ktable_workspace_bytes(Nx, p.nbins, nC, 1);
         if (!ws.alloc(wsb) || !coeff.alloc((size_t)p.nbins * nC * sizeof(double)) ||
             !nxmin.alloc((size_t)p.nbins * sizeof(int32_t)) || !Ks.alloc(Nx * sizeof(double)))
             return fail("Error: k-table allocation failed");
         HKCHK(hk_ktable_prepare(&grid, p.numax, p.nbins, nC, ws.p, wsb));
-        if (p.doResampling > 0 && p.doStoreK > 0)
+        if (need_recon)
             if (!Ky.alloc(Nx * sizeof(double)) || !scr.alloc(Nx * sizeof(double)))
                 return fail("Error: k-table allocation failed");
         FILE *cb = nullptr, *kb = nullptr;
@@ -1073,9 +1170,10 @@ bool run(int argc, char **argv) {
         coeff_h.resize((size_t)p.nbins * nC);
         nxmin_h.resize(p.nbins);
         for (int iP = 0; iP < p.nP; ++iP) {
-            HKCHK(hk_ktable(K_d + (size_t)iP * Nx, 1, &grid, p.numax, p.nbins, nC, p.kmin,
-                            unitScale, coeff.as<double>(), nxmin.as<int32_t>(),
-                            Ks.as<double>(), ws.p, wsb, s));
+            double *KP = K_d + (size_t)iP * Nx;
+            HKCHK(hk_ktable(KP, 1, &grid, p.numax, p.nbins, nC, p.kmin, unitScale,
+                            coeff.as<double>(), nxmin.as<int32_t>(), Ks.as<double>(), ws.p, wsb,
+                            s));
             HIPCHK(hipMemcpyAsync(coeff_h.data(), coeff.p, coeff_h.size() * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(nxmin_h.data(), nxmin.p, nxmin_h.size() * sizeof(int32_t),
@@ -1098,25 +1196,25 @@ bool run(int argc, char **argv) {
                     }
                 }
             }
+            if (!need_sorted && !need_recon) continue;
+            const double *src = Ks.as<double>();
+            if (need_recon) {
+                HKCHK(hk_ktable_reconstruct(coeff.as<double>(), nxmin.as<int32_t>(), 1, p.nbins,
+                                            Nxb, nC, p.kmin, unitScale, Ky.as<double>(),
+                                            scr.as<double>(), s));
+                src = Ky.as<double>();
+            }
+            // K(y) replaces K(x) of this pressure, as in the reference (the
+            // transmission below reads it from there)
+            HIPCHK(hipMemcpyAsync(KP, src, Nx * sizeof(double), hipMemcpyDeviceToDevice, s));
             if (p.doStoreK > 0) {
-                const double *src = Ks.as<double>();
-                if (!need_sorted) {
-                    HKCHK(hk_ktable_reconstruct(coeff.as<double>(), nxmin.as<int32_t>(), 1, p.nbins,
-                                                Nxb, nC, p.kmin, unitScale, Ky.as<double>(),
-                                                scr.as<double>(), s));
-                    src = Ky.as<double>();
-                }
                 HIPCHK(hipMemcpyAsync(K_h.data(), src, Nx * sizeof(double), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 for (int b = 0; b < p.nbins; ++b) {
                     FILE *o = p.doStoreK == 1 ? kb : open_bin(kname, b);
                     if (!o) return fail("Error: cannot open %s output", kname.c_str());
-                    const int64_t il = (int64_t)b * Nxb;
-                    for (int j = 0; j < Nxb; ++j) {
-                        const double y = j / (double)(Nxb - 1);
-                        if (p.nP == 1) std::fprintf(o, "%g %.20g\n", y, K_h[il + j] * unitScale);
-                        else std::fprintf(o, "%g %.20g %g %g %d\n", y, K_h[il + j] * unitScale, p.T, Plist[iP], j);
-                    }
+                    write_bin_K(o, p, K_h.data() + (int64_t)b * Nxb, Nxb, unitScale, edges,
+                                Plist[iP]);
                     if (p.doStoreK == 1) std::fprintf(o, "\n\n");
                     else std::fclose(o);
                 }
@@ -1125,13 +1223,67 @@ bool run(int argc, char **argv) {
         if (cb) std::fclose(cb);
         if (kb) std::fclose(kb);
     }
+    HIPCHK(hipStreamSynchronize(s));
     const double tk = seconds_since(t_kt);
     std::printf("Time for Resampling:   %g seconds\n", tk);
+
+    // transmission functions (heliosk.cu:2373-2449).  With resampling the
+    // reference blends every pressure with the Nxmin of the last pressure
+    // fitted (the per-bin counts left in its buffer); without, Nxmin = 0.
+    auto t_tr = std::chrono::steady_clock::now();
+    if (p.doTransmission > 0) {
+        DevBuf trd;
+        if (!trd.alloc((size_t)p.nbins * p.nTr * sizeof(double)))
+            return fail("Error: allocation for the transmission failed");
+        std::vector<double> tr((size_t)p.nbins * p.nTr);
+        const std::string tname = "Out_" + p.name + "_tr";
+        FILE *o = nullptr;
+        if (p.doTransmission == 1) o = std::fopen((tname + ".dat").c_str(), fmode);
+        if (p.doTransmission == 2 && p.replaceFiles == 1) {
+            for (int b = 0; b < p.nbins; ++b) {
+                char nm[1200];
+                std::snprintf(nm, sizeof(nm), "%s%04d.dat", tname.c_str(), b);
+                FILE *t = std::fopen(nm, "w");
+                if (t) std::fclose(t);
+            }
+        }
+        for (int iP = 0; iP < p.nP; ++iP) {
+            HKCHK(hk_transmission(K_d + (size_t)iP * Nx, p.nbins, (int)Nxb,
+                                  p.doResampling > 0 ? nxmin.as<int32_t>() : nullptr, p.kmin,
+                                  p.nTr, p.dTr, trd.as<double>(), s));
+            HIPCHK(hipMemcpyAsync(tr.data(), trd.p, tr.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int b = 0; b < p.nbins; ++b) {
+                FILE *f = o;
+                if (p.doTransmission == 2) {
+                    char nm[1200];
+                    std::snprintf(nm, sizeof(nm), "%s%04d.dat", tname.c_str(), b);
+                    f = std::fopen(nm, "a");
+                }
+                if (!f) return fail("Error: cannot open %s output", tname.c_str());
+                for (int j = 0; j < p.nTr; ++j) {
+                    const double m = std::exp((j - p.nTr / 2) * p.dTr);
+                    const double v = tr[(size_t)b * p.nTr + j];
+                    if (p.nP == 1) std::fprintf(f, "%.20g %.20g\n", m, v);
+                    else std::fprintf(f, "%.20g %.20g %.20g %.20g %d\n", m, v, p.T, Plist[iP], j);
+                }
+                if (p.doTransmission == 1) std::fprintf(f, "\n\n");
+                else std::fclose(f);
+            }
+        }
+        if (o) std::fclose(o);
+    }
+    const double ttr = seconds_since(t_tr);
+    std::printf("Time for Transmission: %g seconds\n", ttr);
+
     const double tt = seconds_since(t_all);
     std::fprintf(info, "\nTime for input:        %g seconds\n", tm.read);
     std::fprintf(info, "Time for Lines:        %g seconds\n", tm.lines);
     std::fprintf(info, "Time for K(x):         %g seconds\n", tm.voigt);
+    std::fprintf(info, "Time for mean K(x):    %g seconds\n", tmean);
     std::fprintf(info, "Time for Resampling:   %g seconds\n", tk);
+    std::fprintf(info, "Time for Transmission: %g seconds\n", ttr);
     std::fprintf(info, "Total time:            %g seconds\n", tt);
     std::fclose(info);
     HIPCHK(hipStreamDestroy(s));

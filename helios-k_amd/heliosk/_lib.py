@@ -88,6 +88,10 @@ def lib():
         L.hk_lines_sort_workspace_bytes.restype = c_size
         L.hk_lines_sort_workspace_bytes.argtypes = [c_i64]
         L.hk_lines_sort.argtypes = [P(Lines), P(Lines), c_vp, c_size, c_vp]
+        L.hk_means_workspace_bytes.restype = c_size
+        L.hk_means_workspace_bytes.argtypes = []
+        L.hk_means.argtypes = [c_vp, P(Grid), c_d, c_vp, c_vp, c_size, c_vp]
+        L.hk_transmission.argtypes = [c_vp, c_int, c_int, c_vp, c_d, c_int, c_d, c_vp, c_vp]
         L.hk_voigt_workspace_bytes.restype = c_size
         L.hk_voigt_workspace_bytes.argtypes = [P(Grid), P(Tune)]
         L.hk_voigt_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, P(Tune),
@@ -127,7 +131,8 @@ def check(rc):
 
 # Every symbol include/heliosk.h declares.
 EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state_init",
-            "hk_lines_sort_workspace_bytes", "hk_lines_sort",
+            "hk_lines_sort_workspace_bytes", "hk_lines_sort", "hk_means_workspace_bytes",
+            "hk_means", "hk_transmission",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
             "hk_voigt_autotune",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",

@@ -21,7 +21,7 @@ NA = 6.0221412927e23          # define.h:42
 __all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
            "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
-           "ktable_reconstruct", "adjoint", "synth_lines", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
+           "ktable_reconstruct", "adjoint", "synth_lines", "means", "transmission", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
 
 def _dev():
@@ -375,6 +375,30 @@ def opacity(T, P, S, vy, species, grid, Q=None, cut_mode=0, cut=0.0, kmin=0.0):
 def binned_log_mean(K, nbins, w):
     """sum_b w_b log(mean of K over bin b) for nbins equal bins of the grid."""
     return torch.sum(w * torch.log(K.view(nbins, -1).mean(dim=1)))
+
+
+def means(K, grid, T, stream=None):
+    """[sum B K, sum_{K!=0} dB/K, sum B, sum dB] of one spectrum on the grid
+    (device tensor of 4); Planck mean = s[0]/s[2], Rosseland = s[3]/s[1]."""
+    K = K.contiguous()
+    sums = torch.empty(4, dtype=torch.float64, device=K.device)
+    nbytes = lib().hk_means_workspace_bytes()
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
+    check(lib().hk_means(_ptr(K), ctypes.byref(grid), float(T), _ptr(sums), _ptr(ws), nbytes,
+                         _stream(stream)))
+    return sums
+
+
+def transmission(Ks, nbins, Nxb, nTr, dTr, kmin=0.0, nxmin=None, stream=None):
+    """Transmission function (nbins, nTr) of nbins sorted bins of Nxb values."""
+    Ks = Ks.contiguous()
+    Tr = torch.empty(nbins * nTr, dtype=torch.float64, device=Ks.device)
+    nx = None
+    if nxmin is not None:
+        nx = torch.as_tensor(nxmin, dtype=torch.int32).to(Ks.device).contiguous()
+    check(lib().hk_transmission(_ptr(Ks), int(nbins), int(Nxb), None if nx is None else _ptr(nx),
+                                float(kmin), int(nTr), float(dTr), _ptr(Tr), _stream(stream)))
+    return Tr.view(nbins, nTr)
 
 
 def bin_segments(grid, numax, nbins):

@@ -24,6 +24,8 @@
  *                        (heliosk.cu:2046-2193: thrust sorts, findCut,
  *                        rescale, copyK2, lnK, Vandermonde/QR, leastSquare)
  *   hk_ktable_reconstruct expfx_kernel + rescale(f=-1) (heliosk.cu:2189-2193)
+ *   hk_means             Mean_kernel + IntegrateMean_kernel (resample.h:635-771)
+ *   hk_transmission      Integrate_kernel (resample.h:541-614)
  *   hk_voigt_adjoint     no reference counterpart (gradient of the binned
  *                        opacity; see the function comment)
  */
@@ -242,6 +244,27 @@ int hk_ktable_reconstruct(const double *coeff, const int32_t *nxmin, int nspec,
  * c = P[:, :nC] m + P[:, nC:] b0 with m_i = sum_k T_i(x_k) b_k and
  * b0 = b[0:nC]; it reproduces leastSquare_kernel (NC-1 reflections). */
 int hk_ktable_plan_host(int Nxb, int nC, double *P);
+
+/* Planck and Rosseland mean sums of one spectrum on a uniform grid
+ * (Mean_kernel + IntegrateMean_kernel, resample.h:635-771, as main calls
+ * them at heliosk.cu:1953-2000).  With x_j = fma(j, dnu, numin),
+ * B_j the Planck function and dB_j its T derivative (B = dB = 0 at x = 0),
+ * sums (4 doubles, device) = [sum_j B_j K_j, sum_{K_j != 0} dB_j / K_j,
+ * sum_j B_j, sum_j dB_j]; the Planck mean is sums[0]/sums[2], the
+ * Rosseland mean sums[3]/sums[1].  Plain sums, as the reference writes them
+ * (its Simpson end weights are still zero when the means run). */
+size_t hk_means_workspace_bytes(void);
+int hk_means(const double *K, const hk_grid *grid, double T, double *sums,
+             void *workspace, size_t workspace_bytes, hk_stream_t stream);
+
+/* Transmission function of each bin (Integrate_kernel, resample.h:541-614;
+ * heliosk.cu:2400-2440).  Ksorted holds nbins bins of Nxb ascending values
+ * (hk_ktable's Ksorted, or its reconstruction); nxmin (device, NULL = all 0)
+ * the per-bin count of values at kmin.  For t = 0 .. nTr-1,
+ * m = exp((t - nTr/2) dTr) and Tr[b*nTr + t] is the Simpson-corrected mean
+ * of exp(-K m) over the bin, blended with exp(-kmin m) by Nxmin/Nxb. */
+int hk_transmission(const double *Ksorted, int nbins, int Nxb, const int32_t *nxmin,
+                    double kmin, int nTr, double dTr, double *Tr, hk_stream_t stream);
 
 /* Adjoint of the Voigt sum: given gK_j = dL/dK_j on the grid, writes
  * gS[i] = dL/dS_i (S = the raw S_file), gvy[i] = dL/dvy_i and

@@ -43,6 +43,18 @@
  *      NC-1 reflections), 362-388 (findCut), 398-437 (rescale),
  *      472-481 (lnK), 497-524 (expfx).  Pinned by plots/p004 and plots/p005
  *      (Out_r_cbin.dat, Out_r_bin.dat).
+ *  hko_means
+ *      Mean_kernel + IntegrateMean_kernel (resample.h:635-771): the four
+ *      Planck / Rosseland sums, without the Simpson end terms (their weights
+ *      are still zero in the reference when the means run, heliosk.cu:
+ *      1953-2000 before SimpsonCoefficient at 2392).  Pinned by
+ *      plots/p009/Out_p{1,3}_mean.dat and by closed forms (constant K gives
+ *      both means = K; sum B dnu -> sigma T^4 / pi).
+ *  hko_transmission
+ *      Integrate_kernel (resample.h:541-614) with SimpsonCoefficient's end
+ *      weights (resample.h:618-626).  Pinned by plots/p006/Out_i_tr0000.dat
+ *      and _tr0004.dat (from Out_i_bin0000/0004.dat) and by the closed form
+ *      Tr(m) = exp(-K m) for a constant bin.
  *
  * Floating-point contraction: the reference was compiled by nvcc, whose
  * default --fmad=true turns a*b+c into one fused multiply-add.  Where the
@@ -624,4 +636,58 @@ void hko_reconstruct(const double *coeff, const int *Nxmin, int Nxb,
         }
     }
     free(F);
+}
+
+
+/* ---------------------------------------------------------------- means */
+/* resample.h:635-664 (Mean_kernel) and 669-771 (IntegrateMean_kernel). */
+/* Neumaier-compensated running sum: the oracle's sums carry O(eps) error
+ * whatever the order and dynamic range of the terms. */
+static void hko_nsum(double *s, double *c, double v) {
+    double t = *s + v;
+    if (fabs(*s) >= fabs(v)) *c += (*s - t) + v;
+    else *c += (v - t) + *s;
+    *s = t;
+}
+
+void hko_means(const double *K, double numin, double dnu, int64_t Nx, double T,
+               double *sums) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, cmp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < Nx; ++j) {
+        double x = fma((double)j, dnu, numin);
+        double t1 = 2.0 * HKO_H * x * x * x * HKO_C * HKO_C;
+        double t2 = HKO_H * x * HKO_C / (HKO_KB * T);
+        double e = exp(t2);
+        double B = t1 / (e - 1.0);
+        double dB = t1 * t2 / (T * (e - 2.0 + 1.0 / e));
+        if (x == 0.0) { B = 0.0; dB = 0.0; }
+        hko_nsum(&s[0], &cmp[0], B * K[j]);
+        if (K[j] != 0.0) hko_nsum(&s[1], &cmp[1], dB / K[j]);
+        hko_nsum(&s[2], &cmp[2], B);
+        hko_nsum(&s[3], &cmp[3], dB);
+    }
+    for (int q = 0; q < 4; ++q) sums[q] = s[q] + cmp[q];
+}
+
+/* --------------------------------------------------------- transmission */
+/* resample.h:541-614; end weights resample.h:618-626. */
+void hko_transmission(const double *Ks, int nbins, int Nxb, const int32_t *nxmin,
+                      double kmin, int nTr, double dTr, double *Tr) {
+    static const double w[3] = {-5.0 / 8.0, 1.0 / 6.0, -1.0 / 24.0};
+    for (int b = 0; b < nbins; ++b) {
+        const double *k = Ks + (int64_t)b * Nxb;
+        double nmin = nxmin ? (double)nxmin[b] : 0.0;
+        for (int t = 0; t < nTr; ++t) {
+            double m = exp((double)(t - nTr / 2) * dTr);
+            double a = 0.0, ca = 0.0;
+            for (int i = 0; i < Nxb; ++i) hko_nsum(&a, &ca, exp(-k[i] * m));
+            for (int i = 0; i < 3; ++i) {
+                hko_nsum(&a, &ca, w[i] * exp(-k[i] * m));
+                hko_nsum(&a, &ca, w[2 - i] * exp(-k[Nxb - 1 - i] * m));
+            }
+            a += ca;
+            Tr[(int64_t)b * nTr + t] = a / ((double)(Nxb - 1)) * (Nxb - nmin) / ((double)Nxb)
+                                       + exp(-kmin * m) * nmin / ((double)Nxb);
+        }
+    }
 }

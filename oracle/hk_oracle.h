@@ -82,6 +82,16 @@ void hko_reconstruct(const double *coeff, const int *Nxmin, int Nxb,
                      int nbins, int nC, double kmin, double unitScale,
                      double *Ky);
 
+/* ---- consumers of K (resample.h) ---- */
+/* Planck / Rosseland mean sums [sum B K, sum_{K!=0} dB/K, sum B, sum dB] on
+ * x_j = fma(j, dnu, numin) (resample.h:635-771, plain sums). */
+void hko_means(const double *K, double numin, double dnu, int64_t Nx, double T,
+               double *sums);
+/* Transmission of each of nbins sorted bins of Nxb values
+ * (resample.h:541-614); nxmin may be NULL (all 0). */
+void hko_transmission(const double *Ks, int nbins, int Nxb, const int32_t *nxmin,
+                      double kmin, int nTr, double dTr, double *Tr);
+
 #ifdef __cplusplus
 }
 #endif

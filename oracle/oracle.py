@@ -51,6 +51,8 @@ def lib():
         L.hko_ktable.argtypes = [_D, c_i64, c_int, c_int, c_int, c_d, c_d, _D, _INT]
         L.hko_lsq.argtypes = [c_int, c_int, _D, _D]
         L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
+        L.hko_means.argtypes = [_D, c_d, c_d, c_i64, c_d, _D]
+        L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
         _lib = L
     return _lib
 
@@ -199,3 +201,22 @@ def reconstruct(coeff, nxmin, Nxb, kmin, unit_scale=1.0):
     lib().hko_reconstruct(_p(coeff), _p(nx, _INT), int(Nxb), int(nbins), int(nC),
                           float(kmin), float(unit_scale), _p(Ky))
     return Ky
+
+
+def means(K, numin, dnu, T):
+    """[sum B K, sum_{K!=0} dB/K, sum B, sum dB] on x_j = fma(j, dnu, numin)."""
+    K = _f64(K)
+    out = np.empty(4)
+    lib().hko_means(_p(K), float(numin), float(dnu), int(K.size), float(T), _p(out))
+    return out
+
+
+def transmission(Ks, nbins, Nxb, nTr, dTr, kmin=0.0, nxmin=None):
+    """Tr[b, t] of nbins sorted bins of Nxb values."""
+    Ks = _f64(Ks)
+    assert Ks.size >= nbins * Nxb
+    nx = None if nxmin is None else np.ascontiguousarray(nxmin, dtype=np.int32)
+    Tr = np.empty(nbins * nTr)
+    lib().hko_transmission(_p(Ks), int(nbins), int(Nxb), None if nx is None else _p(nx, _I32),
+                           float(kmin), int(nTr), float(dTr), _p(Tr))
+    return Tr.reshape(nbins, nTr)

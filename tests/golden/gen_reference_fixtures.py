@@ -16,6 +16,9 @@ It writes data only (numbers read from the text outputs, no code):
   p005_Out_r_bin.dat       plots/p005/Out_r_bin.dat (reconstruction)
   p009_p{1,3}_K.npz        K columns of plots/p009/Out_p{1,3}.dat (1e5 pts, 5 bins)
   p009_p{1,3}_sorted.sha256  sha256 of the float64 K columns of Out_p*_bin000{0..4}.dat
+  p009_p{1,3}_mean.dat     plots/p009/Out_p{1,3}_mean.dat (Planck / Rosseland means)
+  p006_bin{0,4}.npz        K column of plots/p006/Out_i_bin000{0,4}.dat (sorted K, 60000 pts)
+  p006_tr{0,4}.npy         plots/p006/Out_i_tr000{0,4}.dat (m, transmission), 1000 rows
 """
 import hashlib
 import os
@@ -48,6 +51,14 @@ def main(ref):
             h.update(kcol(os.path.join(p, "p009", "Out_%s_bin%04d.dat" % (run, b))).tobytes())
         with open(os.path.join(HERE, "p009_%s_sorted.sha256" % run), "w") as f:
             f.write(h.hexdigest() + "\n")
+    for run in ("p1", "p3"):
+        shutil.copyfile(os.path.join(p, "p009", "Out_%s_mean.dat" % run),
+                        os.path.join(HERE, "p009_%s_mean.dat" % run))
+    for b in (0, 4):
+        np.savez_compressed(os.path.join(HERE, "p006_bin%d.npz" % b),
+                            K=kcol(os.path.join(p, "p006", "Out_i_bin%04d.dat" % b)))
+        np.save(os.path.join(HERE, "p006_tr%d.npy" % b),
+                np.loadtxt(os.path.join(p, "p006", "Out_i_tr%04d.dat" % b), dtype=np.float64))
     print("fixtures written to", HERE)
 
 

@@ -67,7 +67,6 @@ def _setup(tmp_path, **kw):
     (dict(profile=2), "Error: profile 2 is not supported"),
     (dict(numin=0.0, numax=10.0, dnu=0.01, nbins=7), "Error: range cannot be divided evenly in bins."),
     (dict(ciaSystem="H2-H2"), "Error: CIA systems are not supported"),
-    (dict(doTransmission=1), "Error: transmission functions are not supported"),
     (dict(binsFile="bins.dat"), "Error: bins files are not supported"),
 ])
 def test_driver_rejects_like_the_reference(tmp_path, kw, msg):
@@ -75,6 +74,42 @@ def test_driver_rejects_like_the_reference(tmp_path, kw, msg):
     rc, out = run_driver(str(tmp_path))
     assert rc == 0                         # the reference exits 0 after "Error: ..."
     assert msg in out, out
+
+
+def test_driver_rejects_unordered_output_edges(tmp_path):
+    (tmp_path / "edges.txt").write_text("0.0\n0.5\n0.4\n1.0\n")
+    _setup(tmp_path, OutputEdgesFile="edges.txt", doStoreSK=1)
+    rc, out = run_driver(str(tmp_path))
+    assert "Use output edges in file edges.txt" in out
+    assert "Error; output edges not monotonic growing" in out, out
+    _setup(tmp_path, OutputEdgesFile="missing.txt", doStoreSK=1)
+    rc, out = run_driver(str(tmp_path))
+    assert "Error: output edges file not found: missing.txt" in out, out
+
+
+def edge_means(y, K, edges):
+    """Out_<name>_bin rows with an output-edges file (heliosk.cu:2266-2281):
+    the running mean restarts at each edge e with y_j <= e <= y_{j+1} (point j
+    included); the first edge opens the first interval."""
+    rows, ie, n, acc = [], 0, 0, 0.0
+    N = len(K)
+    for j in range(N):
+        y0, y1 = j / (N - 1.0), (j + 1) / (N - 1.0)
+        n += 1
+        acc += K[j]
+        if ie < len(edges) and y0 <= edges[ie] <= y1:
+            if ie > 0:
+                rows.append((0.5 * (edges[ie] + edges[ie - 1]), acc / n))
+            ie += 1
+            n, acc = 0, 0.0
+    return rows
+
+
+def test_edge_means_hand_case():
+    K = np.arange(11, dtype=float) ** 2
+    rows = edge_means(np.linspace(0, 1, 11), K, [0.0, 0.25, 0.5, 1.0])
+    assert rows == [(0.125, (1 + 4) / 2), (0.375, (9 + 16) / 2),
+                    (0.75, (25 + 36 + 49 + 64 + 81) / 5)]
 
 
 def test_driver_undefined_key(tmp_path):

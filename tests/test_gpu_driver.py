@@ -296,3 +296,104 @@ def test_driver_chunked_stream_matches_oracle(tmp_path):
     e1, e2, e12 = rel_err(K1, Ko), rel_err(K2, Ko), rel_err(K1, K2)
     print("chunked %.3e, one chunk %.3e, between %.3e" % (e1, e2, e12))
     assert e1 <= 1e-11 and e2 <= 1e-11 and e12 <= 1e-13
+
+
+def _read_rows(path):
+    return [l.split() for l in open(path) if l.strip()]
+
+
+def test_driver_means_transmission_and_bins(tmp_path):
+    """doMean = 1, doTransmission = 2 and doStoreSK = 1 without resampling:
+    means of K(x) and the transmission of each sorted bin, against the oracle
+    applied to the driver's own K (text outputs carry all digits) and to the
+    oracle's K."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 3000, 41, 995.0, 1015.0)
+    T, P, cut, nTr, dTr = 1100.0, 0.8, 5.0, 300, 0.07
+    F.write_param_dat(str(tmp_path / "param.dat"), name="t", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=1, nbins=NBINS,
+                      doMean=1, doTransmission=2, nTr=nTr, dTr=dTr, doStoreSK=1)
+    out = run_driver(str(tmp_path))
+    assert "Time for Transmission" in out and "Time for mean K(x)" in out
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    Nxb = nx // NBINS
+    Kd = F.read_out_dat(str(tmp_path / "Out_t.dat"))[0][:, 1]
+    from test_oracle import mean_lines
+    means = np.loadtxt(tmp_path / "Out_t_mean.dat")
+    np.testing.assert_allclose(means, mean_lines(O.means(Kd, NUMIN, DNU, T), T, DNU), rtol=1e-13)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    Ko = oracle_K([sp], [1.0], meanMass, T, P, 1.0, 1.0, cut, NUMIN, DNU, nx, 0.0)
+    np.testing.assert_allclose(means, mean_lines(O.means(Ko, NUMIN, DNU, T), T, DNU), rtol=1e-11)
+    # K(y) per bin = the sorted bins of K(x)
+    kb = F.read_out_dat(str(tmp_path / "Out_t_bin.dat"))
+    Ky = np.concatenate([b[:, 1] for b in kb])
+    np.testing.assert_array_equal(Ky, np.concatenate([np.sort(Kd[b * Nxb:(b + 1) * Nxb])
+                                                      for b in range(NBINS)]))
+    To = O.transmission(Ky, NBINS, Nxb, nTr, dTr)
+    for b in range(NBINS):
+        tr = np.loadtxt(tmp_path / ("Out_t_tr%04d.dat" % b))
+        assert tr.shape == (nTr, 2)
+        np.testing.assert_allclose(tr[:, 0], np.exp((np.arange(nTr) - nTr // 2) * dTr), rtol=1e-15)
+        np.testing.assert_allclose(tr[:, 1], To[b], rtol=1e-13)
+
+
+def test_driver_resampled_transmission_uses_last_nxmin(tmp_path):
+    """Resampling, two pressures and doTransmission = 1: the transmission of
+    each pressure is taken from its reconstructed K(y) and blended with the
+    Nxmin of the last pressure fitted (heliosk.cu:2130-2193, 2400-2410).
+    Cut in Lorentz widths makes the empty part of each bin depend on P."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 400, 43, 1000.0, 1004.0)
+    T, cut, nTr, dTr, nC, kmin = 700.0, 30.0, 64, 0.3, 12, 1e-28
+    (tmp_path / "P.txt").write_text("0.05\n1.5\n")
+    F.write_param_dat(str(tmp_path / "param.dat"), name="r", T=T, P=1.0, PFile="P.txt",
+                      Species_Name="01_tst", numin=NUMIN, numax=NUMAX, dnu=DNU, cutMode=1,
+                      cut=cut, doStoreFullK=0, nbins=NBINS, doResampling=1, nC=nC, kmin=kmin,
+                      doTransmission=1, nTr=nTr, dTr=dTr)
+    run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    Nxb = nx // NBINS
+    cb = F.read_cbin(str(tmp_path / "Out_r_cbin.dat"), nC)
+    # ystart = min(Nxmin / (Nxb - 1), 1); a bin of kmin only (Nxmin = Nxb) prints zeros
+    empty = (cb["ystart"] == 1.0) & np.all(cb["coeff"] == 0.0, axis=1)
+    nxmin = np.where(empty, Nxb, np.rint(cb["ystart"] * (Nxb - 1))).astype(np.int32)
+    nxmin = nxmin.reshape(2, NBINS)
+    assert not np.array_equal(nxmin[0], nxmin[1]), nxmin      # the quirk is visible
+    coeff = cb["coeff"].reshape(2, NBINS, nC)
+    rows = np.array([[float(v) for v in r] for r in _read_rows(tmp_path / "Out_r_tr.dat")])
+    assert rows.shape == (2 * NBINS * nTr, 5)
+    for iP in range(2):
+        Ky = O.reconstruct(coeff[iP], nxmin[iP], Nxb, kmin)
+        To = O.transmission(Ky, NBINS, Nxb, nTr, dTr, kmin=kmin, nxmin=nxmin[1])
+        got = rows[iP * NBINS * nTr:(iP + 1) * NBINS * nTr, 1].reshape(NBINS, nTr)
+        np.testing.assert_allclose(got, To, rtol=1e-12)
+        np.testing.assert_array_equal(rows[iP * NBINS * nTr:(iP + 1) * NBINS * nTr, 4],
+                                      np.tile(np.arange(nTr), NBINS))
+
+
+def test_driver_output_edges(tmp_path):
+    """doStoreSK = 2 with an output-edges file, two pressures: the per-bin
+    rows are the edge-interval means of the plain K(y) rows."""
+    from test_driver_cpu import edge_means
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 1500, 47, 995.0, 1015.0)
+    (tmp_path / "P.txt").write_text("0.3\n2.0\n")
+    edges = [0.0, 0.1, 0.35, 0.6, 0.85, 0.999999, 1.0]
+    (tmp_path / "edges.txt").write_text("".join("%r\n" % e for e in edges))
+    common = dict(name="e", T=900.0, P=1.0, PFile="P.txt", Species_Name="01_tst", numin=NUMIN,
+                  numax=NUMAX, dnu=DNU, cut=5.0, doStoreFullK=0, nbins=NBINS, doStoreSK=2)
+    F.write_param_dat(str(tmp_path / "param.dat"), **common)
+    run_driver(str(tmp_path))
+    plain = [np.array([[float(v) for v in r] for r in _read_rows(tmp_path / ("Out_e_bin%04d.dat" % b))])
+             for b in range(NBINS)]
+    F.write_param_dat(str(tmp_path / "param.dat"), OutputEdgesFile="edges.txt", **common)
+    out = run_driver(str(tmp_path))
+    assert "Use output edges in file edges.txt" in out
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    Nxb = nx // NBINS
+    for b in range(NBINS):
+        got = [[float(v) for v in r] for r in _read_rows(tmp_path / ("Out_e_bin%04d.dat" % b))]
+        expect = []
+        for iP, P in enumerate((0.3, 2.0)):
+            Kb = plain[b][iP * Nxb:(iP + 1) * Nxb, 1]
+            for k, (mid, v) in enumerate(edge_means(None, Kb, edges)):
+                expect.append([float("%g" % mid), v, 900.0, P, k])
+        assert len(got) == len(expect) == 2 * (len(edges) - 1)
+        np.testing.assert_allclose(np.array(got), np.array(expect), rtol=1e-15)

@@ -359,3 +359,68 @@ def test_findcut_rules():
     K[:10] = np.linspace(1, 2, 10)
     _, nxmin = O.ktable(K.copy(), Nxb, nbins, 4, 0.0)
     assert nxmin[0] == 0
+
+
+# ---------------------------------------------------------------- means and transmission
+SIGMA = (2 * 1.3806489e-16 ** 4 / (6.62606957e-27 ** 3 * 2.99792458e10 ** 2 * 15.0)) * math.pi ** 5
+
+
+def mean_lines(sums, T, dnu):
+    """The six numbers of Out_<name>_mean.dat for one pressure
+    (heliosk.cu:1976-1999)."""
+    return [sums[0] / sums[2], sums[3] / sums[1], sums[2] * dnu, SIGMA * T ** 4 / math.pi,
+            sums[3] * dnu, 4.0 * SIGMA * T ** 3 / math.pi]
+
+
+@pytest.mark.parametrize("run", ["p1", "p3"])
+def test_means_match_p009(golden_dir, run):
+    K = np.load(os.path.join(golden_dir, "p009_%s_K.npz" % run))["K"]
+    ref = np.loadtxt(os.path.join(golden_dir, "p009_%s_mean.dat" % run))
+    got = mean_lines(O.means(K, 24400.0, 0.001, 1500.0), 1500.0, 0.001)
+    np.testing.assert_allclose(got, ref, rtol=2e-14, atol=0)
+
+
+def test_means_closed_forms():
+    T = 800.0
+    K = np.full(600000, 3.5e-21)
+    s = O.means(K, 0.0, 0.05, T)            # x from 0 to 3e4 cm^-1: the whole Planck curve
+    assert abs(s[0] / s[2] - 3.5e-21) <= 1e-12 * 3.5e-21     # constant K: both means are K
+    assert abs(s[3] / s[1] - 3.5e-21) <= 1e-12 * 3.5e-21
+    assert abs(s[2] * 0.05 / (SIGMA * T ** 4 / math.pi) - 1) <= 1e-9       # int B = sigma T^4/pi
+    assert abs(s[3] * 0.05 / (4 * SIGMA * T ** 3 / math.pi) - 1) <= 1e-9   # int dB/dT
+    K[::7] = 0.0                             # zeros drop out of the Rosseland sum only
+    s2 = O.means(K, 0.0, 0.05, T)
+    assert s2[1] < s[1] and s2[2] == s[2] and s2[3] == s[3]
+
+
+@pytest.mark.parametrize("b", [0, 4])
+def test_transmission_matches_p006(golden_dir, b):
+    Ks = np.load(os.path.join(golden_dir, "p006_bin%d.npz" % b))["K"]
+    ref = np.load(os.path.join(golden_dir, "p006_tr%d.npy" % b))
+    Tr = O.transmission(Ks, 1, Ks.size, 1000, 0.05)[0]
+    np.testing.assert_allclose(np.exp((np.arange(1000) - 500) * 0.05), ref[:, 0], rtol=1e-15)
+    np.testing.assert_allclose(Tr, ref[:, 1], rtol=2e-13, atol=0)
+
+
+def test_transmission_closed_forms():
+    nTr, dTr, N = 40, 0.2, 50
+    m = np.exp((np.arange(nTr) - nTr // 2) * dTr)
+    Ks = np.repeat([0.3, 2.0, 7.0], N)
+    Tr = O.transmission(Ks, 3, N, nTr, dTr)
+    # a constant bin: the end weights sum to -1, so Tr = exp(-K m) exactly
+    np.testing.assert_allclose(Tr, np.exp(-np.outer([0.3, 2.0, 7.0], m)), rtol=1e-14)
+    nmin = np.array([0, 10, 50], dtype=np.int32)
+    Tr = O.transmission(Ks, 3, N, nTr, dTr, kmin=0.1, nxmin=nmin)
+    expect = (np.exp(-np.outer([0.3, 2.0, 7.0], m)) * ((N - nmin) / N)[:, None]
+              + np.exp(-0.1 * m)[None, :] * (nmin / N)[:, None])
+    np.testing.assert_allclose(Tr, expect, rtol=1e-14)
+    # the end weights, point by point: the first point carries -5/8, the last
+    # -1/24 (weights w_i at i and w_{2-i} at N-1-i, resample.h:591-593)
+    f = np.exp(-m)
+    K1 = np.full(N, 1.0); K1[0] = 0.0
+    t1 = O.transmission(K1, 1, N, nTr, dTr)[0]
+    np.testing.assert_allclose(t1, (3 / 8 + (N - 1 - 3 / 8) * f) / (N - 1), rtol=1e-13)
+    K2 = np.full(N, 1.0); K2[-1] = 0.0
+    t2 = O.transmission(K2, 1, N, nTr, dTr)[0]
+    np.testing.assert_allclose(t2, (23 / 24 + (N - 1 - 1 / 2 + 1 / 6 - 5 / 8) * f) / (N - 1),
+                               rtol=1e-13)
