@@ -28,8 +28,8 @@
 // (OutputEdgesFile).
 //
 // Options the reference has that this driver does not implement are
-// rejected with an error message: profiles other than 1 (Voigt), CIA, bins
-// files, "Nnu per bin", sub-Lorentzian wings and removePlinth.
+// rejected with an error message: CIA, bins files, "Nnu per bin" and
+// sub-Lorentzian wings.  Profiles 1-4 and removePlinth are supported.
 //
 // Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
@@ -700,8 +700,7 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
     const size_t stb = hk_line_state_bytes(cap);
     const size_t sortb = hk_lines_sort_workspace_bytes(cap);
     if (sortb == 0) return fail("Error: %s", hk_last_error());
-    hk_tune tune = {0, 0};
-    const size_t wsb = hk_voigt_workspace_bytes(&grid, &tune);
+    const size_t wsb = hk_profile_workspace_bytes(&grid, cap);
     if (!dst.alloc(stb) || !dsws.alloc(sortb) || !dws.alloc(wsb))
         return fail("Error: device allocation of the line state and workspaces failed");
 
@@ -730,6 +729,8 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
     };
     hk_cut c = {p.cutMode, p.cut};
     const int mode = p.precision == 1 ? HK_VOIGT_FP32C : HK_VOIGT_FP64;
+    // removePlinth acts only on cut lines (heliosk.cu:1178)
+    const int plinth = p.removePlinth == 1 && p.cut != 0.0;
 
     // timing events: [sort start, params start, voigt start, voigt end] per step
     std::vector<hipEvent_t> ev;
@@ -782,8 +783,8 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
             HKCHK(hk_line_params(&srt, &tp, &c, &st, s));
             mk.voi.push_back(ev.size());
             if (!event(e)) { ok = false; break; }
-            HKCHK(hk_voigt_accumulate(&st, &grid, p.kmin, 1, mode, &tune, K_d + iP * grid.nx,
-                                      dws.p, wsb, s));
+            HKCHK(hk_profile_accumulate(&st, &grid, p.kmin, 1, p.profile, plinth, mode,
+                                        K_d + iP * grid.nx, dws.p, wsb, s));
             mk.end.push_back(ev.size());
             if (!event(e)) { ok = false; break; }
         }
@@ -908,8 +909,9 @@ bool run(int argc, char **argv) {
     Param p;
     if (!read_parameters(p, "param.dat", argc, argv)) return false;
     // options outside this driver's scope
-    if (p.profile != 1) return fail("Error: profile %d is not supported (Voigt profile = 1 only)", p.profile);
-    if (p.removePlinth) return fail("Error: removePlinth is not supported");
+    if (p.profile < 1 || p.profile > 4) return fail("Error: profile %d is not supported", p.profile);
+    if (p.removePlinth == 1 && p.profile == 4)
+        return fail("Error, remove plinth is not supported for profile 4");   // heliosk.cu:410-413
     if (p.subLorentzian != "-") return fail("Error: sub-Lorentzian files are not supported");
     if (p.ciaSystem != "-") return fail("Error: CIA systems are not supported");
     if (p.bins != "-") return fail("Error: bins files are not supported");

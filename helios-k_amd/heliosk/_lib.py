@@ -13,6 +13,7 @@ HK_OK = 0
 HK_FORM_HITRAN, HK_FORM_EXOMOL, HK_FORM_SUPERLINE = 0, 1, 2
 HK_VOIGT_FP64 = 0
 HK_VOIGT_FP32C = 1
+HK_PROFILE_VOIGT, HK_PROFILE_LORENTZ, HK_PROFILE_DOPPLER, HK_PROFILE_BINNED_GAUSS = 1, 2, 3, 4
 HK_MAX_ISO = 16
 
 c_d = ctypes.c_double
@@ -88,6 +89,10 @@ def lib():
         L.hk_lines_sort_workspace_bytes.restype = c_size
         L.hk_lines_sort_workspace_bytes.argtypes = [c_i64]
         L.hk_lines_sort.argtypes = [P(Lines), P(Lines), c_vp, c_size, c_vp]
+        L.hk_profile_workspace_bytes.restype = c_size
+        L.hk_profile_workspace_bytes.argtypes = [P(Grid), c_i64]
+        L.hk_profile_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, c_int, c_int,
+                                            c_vp, c_vp, c_size, c_vp]
         L.hk_means_workspace_bytes.restype = c_size
         L.hk_means_workspace_bytes.argtypes = []
         L.hk_means.argtypes = [c_vp, P(Grid), c_d, c_vp, c_vp, c_size, c_vp]
@@ -132,7 +137,7 @@ def check(rc):
 # Every symbol include/heliosk.h declares.
 EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state_init",
             "hk_lines_sort_workspace_bytes", "hk_lines_sort", "hk_means_workspace_bytes",
-            "hk_means", "hk_transmission",
+            "hk_means", "hk_transmission", "hk_profile_workspace_bytes", "hk_profile_accumulate",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
             "hk_voigt_autotune",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",

@@ -12,7 +12,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (HK_FORM_EXOMOL, HK_FORM_HITRAN, HK_FORM_SUPERLINE, HK_VOIGT_FP32C,
+from ._lib import (HK_FORM_EXOMOL, HK_FORM_HITRAN, HK_FORM_SUPERLINE, HK_PROFILE_BINNED_GAUSS,
+                   HK_PROFILE_DOPPLER, HK_PROFILE_LORENTZ, HK_PROFILE_VOIGT, HK_VOIGT_FP32C,
                    HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineState, Lines, Spectrum, TP, Tune,
                    check, lib)
 
@@ -21,7 +22,7 @@ NA = 6.0221412927e23          # define.h:42
 __all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
            "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
-           "ktable_reconstruct", "adjoint", "synth_lines", "means", "transmission", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
+           "ktable_reconstruct", "adjoint", "synth_lines", "means", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
 
 def _dev():
@@ -199,6 +200,20 @@ def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, acc
     check(lib().hk_voigt_accumulate(ctypes.byref(state.st), ctypes.byref(grid), float(kmin),
                                     int(bool(accumulate)), int(mode), ctypes.byref(tune),
                                     _ptr(K), _ptr(ws), ws.numel(), _stream(stream)))
+    return K
+
+
+def profile(state, grid, profile, remove_plinth=False, kmin=0.0, voigt_mode=HK_VOIGT_FP64, K=None,
+            accumulate=False, stream=None):
+    """K_j = kmin + sum_i profile_i(x_j) (minus each line's plinth) for
+    profile 1 (Voigt), 2 (Lorentz), 3 (Doppler) or 4 (binned Gaussian)."""
+    if K is None:
+        K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
+    nbytes = lib().hk_profile_workspace_bytes(ctypes.byref(grid), state.nlines)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
+    check(lib().hk_profile_accumulate(ctypes.byref(state.st), ctypes.byref(grid), float(kmin),
+                                      int(bool(accumulate)), int(profile), int(bool(remove_plinth)),
+                                      int(voigt_mode), _ptr(K), _ptr(ws), nbytes, _stream(stream)))
     return K
 
 

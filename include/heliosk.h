@@ -24,6 +24,8 @@
  *                        (heliosk.cu:2046-2193: thrust sorts, findCut,
  *                        rescale, copyK2, lnK, Vandermonde/QR, leastSquare)
  *   hk_ktable_reconstruct expfx_kernel + rescale(f=-1) (heliosk.cu:2189-2193)
+ *   hk_profile_accumulate Line6fA/B/C_kernel<profile 1-4> + Plinth_kernel
+ *                        (voigt.h:1091-1262)
  *   hk_means             Mean_kernel + IntegrateMean_kernel (resample.h:635-771)
  *   hk_transmission      Integrate_kernel (resample.h:541-614)
  *   hk_voigt_adjoint     no reference counterpart (gradient of the binned
@@ -55,6 +57,14 @@ enum {
 enum {
     HK_VOIGT_FP64 = 0,          /* plain FP64 profile, FP64 accumulation */
     HK_VOIGT_FP32C = 1          /* FP32 profile, compensated FP32 tile sums */
+};
+
+/* Line profiles (param.dat "profile", voigt.h:1235-1262). */
+enum {
+    HK_PROFILE_VOIGT = 1,
+    HK_PROFILE_LORENTZ = 2,
+    HK_PROFILE_DOPPLER = 3,
+    HK_PROFILE_BINNED_GAUSS = 4     /* Gaussian averaged over each grid cell */
 };
 
 /* Line-list forms: how the file strength and widths become S(T), alpha_L(T,P).
@@ -244,6 +254,26 @@ int hk_ktable_reconstruct(const double *coeff, const int32_t *nxmin, int nspec,
  * c = P[:, :nC] m + P[:, nC:] b0 with m_i = sum_k T_i(x_k) b_k and
  * b0 = b[0:nC]; it reproduces leastSquare_kernel (NC-1 reflections). */
 int hk_ktable_plan_host(int Nxb, int nC, double *P);
+
+/* Sum of one line list with any profile, optionally with each line's
+ * plinth removed (Line6fA/B/C_kernel<profile> and Plinth_kernel,
+ * voigt.h:1091-1262; S1 as S3_kernel forms it, voigt.h:458-470).  In Doppler
+ * units x = (x_j - nu)/alpha_D, per pair inside the cut:
+ *   HK_PROFILE_VOIGT         S Re w(x, y)/sqrt(pi)  (hk_voigt_accumulate in
+ *                            voigt_mode, HK_VOIGT_FP64 or HK_VOIGT_FP32C)
+ *   HK_PROFILE_LORENTZ       (S y/pi) / (x^2 + y^2)
+ *   HK_PROFILE_DOPPLER       (S/sqrt(pi)) exp(-x^2)
+ *   HK_PROFILE_BINNED_GAUSS  (S/vb) (erf(x + vb/2) - erf(x - vb/2))/2, vb = dnu/alpha_D
+ * remove_plinth subtracts from every pair the line's profile at its cut
+ * (profiles 1-3; the reference rejects it for profile 4, and so does this
+ * call).  Profiles 2-4 and the plinth are FP64.  Workspace: the Voigt
+ * workspace or the tile table, whichever is larger, plus one double per
+ * line. */
+size_t hk_profile_workspace_bytes(const hk_grid *grid, int64_t nlines);
+int hk_profile_accumulate(const hk_line_state *st, const hk_grid *grid, double kmin,
+                          int accumulate, int profile, int remove_plinth, int voigt_mode,
+                          double *K, void *workspace, size_t workspace_bytes,
+                          hk_stream_t stream);
 
 /* Planck and Rosseland mean sums of one spectrum on a uniform grid
  * (Mean_kernel + IntegrateMean_kernel, resample.h:635-771, as main calls

@@ -32,6 +32,14 @@
  *      1616-1706) onto InitialK's kmin (voigt.h:713-722).  Pinned by
  *      linearity, translation, symmetry and the normalisation
  *      sum_j K_j dnu -> sum_i S_i for an uncut line.
+ *  hko_profile_value, hko_profile_points
+ *      Profiles 2-4 (Lorentz, Doppler, binned Gaussian; voigt.h:1235-1262
+ *      with the S1 of S3_kernel, voigt.h:458-470) and the plinth removal
+ *      of Plinth_kernel (voigt.h:1091-1164).  Pinned by closed forms: the
+ *      binned Gaussian sums to S over the grid exactly (the erf differences
+ *      telescope), the Doppler and Lorentz sums approach S (and the
+ *      Lorentz tail's S y / (pi x)), the plinth-removed profile vanishes at
+ *      the cut and equals the plain one minus a constant per line.
  *  hko_voigt_grad
  *      Analytic gradient of sum_j gK_j K_j; pinned by central finite
  *      differences of hko_voigt_points.
@@ -292,6 +300,69 @@ void hko_voigt_points(int64_t nl, const double *nu, const double *S,
             sum = s2;
         }
         K[p] = sum + comp;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Line profiles 1-4 (voigt.h:1235-1262 with S1 of S3_kernel,          */
+/* voigt.h:458-470) and the plinth (Plinth_kernel, voigt.h:1091-1164):  */
+/* per pair, in Doppler units x = (x_j - nu) / alpha_D:                 */
+/*   1 Voigt        S Re w(x, y) / sqrt(pi)                             */
+/*   2 Lorentz      (S y / pi) / (x^2 + y^2)                            */
+/*   3 Doppler      (S / sqrt(pi)) exp(-x^2)                            */
+/*   4 binned Gauss S (1/vb) (erf(x + vb/2) - erf(x - vb/2)) / 2,       */
+/*                  vb = dnu / alpha_D                                  */
+/* minus, with remove_plinth, the profile 1-3 at the cut x_c = hw /     */
+/* alpha_D (the plinth of the truncated line).                          */
+/* ------------------------------------------------------------------ */
+double hko_profile_value(int profile, double S, double x, double y, double vb)
+{
+    double re, im;
+    switch (profile) {
+    case 1: hko_faddeeva(x, y, &re, &im); return S * re * HKO_ISQRTPI;
+    case 2: return S * y / M_PI / (x * x + y * y);
+    case 3: return S * HKO_ISQRTPI * exp(-x * x);
+    default: {
+        /* (erf(x + h) - erf(x - h)) / 2 from erfc on the side away from the
+           centre (the same value, without the cancellation in the wings) */
+        double h = 0.5 * vb;
+        double d = x >= 0.0 ? erfc(x - h) - erfc(x + h) : erfc(-x - h) - erfc(-x + h);
+        return S / vb * 0.5 * d;
+    }
+    }
+}
+
+void hko_profile_points(int64_t nl, const double *nu, const double *S,
+                        const double *ialphaD, const double *y,
+                        const double *hw, double R,
+                        double numin, double dnu, int64_t Nx, double kmin,
+                        int profile, int remove_plinth, double *K, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t j = 0; j < Nx; ++j) {
+        double xj = fma((double)j, dnu, numin);
+        int64_t i0 = lower_bound(nu, nl, xj - R);
+        int64_t i1 = lower_bound(nu, nl, xj + R);
+        if (i1 < nl) ++i1;
+        double sum = kmin, comp = 0.0;
+        for (int64_t i = i0; i < i1; ++i) {
+            double d = xj - nu[i];
+            if (!(d < hw[i] && d > -hw[i])) continue;
+            double vb = dnu * ialphaD[i];
+            double t = hko_profile_value(profile, S[i], d * ialphaD[i], y[i], vb);
+            if (remove_plinth)
+                t -= hko_profile_value(profile, S[i], hw[i] * ialphaD[i], y[i], vb);
+            double s2 = sum + t;
+            if (fabs(sum) >= fabs(t)) comp += (sum - s2) + t;
+            else comp += (t - s2) + sum;
+            sum = s2;
+        }
+        K[j] = sum + comp;
     }
 }
 

@@ -52,6 +52,10 @@ def lib():
         L.hko_lsq.argtypes = [c_int, c_int, _D, _D]
         L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
         L.hko_means.argtypes = [_D, c_d, c_d, c_i64, c_d, _D]
+        L.hko_profile_value.restype = c_d
+        L.hko_profile_value.argtypes = [c_int, c_d, c_d, c_d, c_d]
+        L.hko_profile_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d, c_int,
+                                                              c_int, _D, c_int])
         L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
         _lib = L
     return _lib
@@ -109,6 +113,27 @@ def line_params(lines, T, P, cut_mode, cut, form=0):
                           float(cut), int(form),
                           *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
     return out
+
+
+def profile_points(state, numin, dnu, Nx, profile, remove_plinth=False, kmin=0.0, nthreads=0):
+    """K on all Nx points for profile 1 (Voigt), 2 (Lorentz), 3 (Doppler) or
+    4 (binned Gaussian), optionally with each line's plinth removed."""
+    cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
+    if cols[0].size > 1 and np.any(np.diff(cols[0]) < 0):
+        o = np.argsort(cols[0], kind="stable")
+        cols = [np.ascontiguousarray(c[o]) for c in cols]
+    nl = cols[0].size
+    hw = cols[4]
+    R = float(hw[np.isfinite(hw)].max()) if nl else 0.0
+    K = np.empty(int(Nx))
+    lib().hko_profile_points(nl, *[_p(c) for c in cols], R, float(numin), float(dnu), int(Nx),
+                             float(kmin), int(profile), int(bool(remove_plinth)), _p(K),
+                             int(nthreads))
+    return K
+
+
+def profile_value(profile, S, x, y, vb):
+    return lib().hko_profile_value(int(profile), float(S), float(x), float(y), float(vb))
 
 
 def window(nu, hw, numin, dnu, Nx):

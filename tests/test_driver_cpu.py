@@ -64,7 +64,8 @@ def _setup(tmp_path, **kw):
 
 
 @pytest.mark.parametrize("kw, msg", [
-    (dict(profile=2), "Error: profile 2 is not supported"),
+    (dict(profile=5), "Error: profile 5 is not supported"),
+    (dict(profile=4, removePlinth=1), "Error, remove plinth is not supported for profile 4"),
     (dict(numin=0.0, numax=10.0, dnu=0.01, nbins=7), "Error: range cannot be divided evenly in bins."),
     (dict(ciaSystem="H2-H2"), "Error: CIA systems are not supported"),
     (dict(binsFile="bins.dat"), "Error: bins files are not supported"),

@@ -397,3 +397,28 @@ def test_driver_output_edges(tmp_path):
                 expect.append([float("%g" % mid), v, 900.0, P, k])
         assert len(got) == len(expect) == 2 * (len(edges) - 1)
         np.testing.assert_allclose(np.array(got), np.array(expect), rtol=1e-15)
+
+
+@pytest.mark.parametrize("profile, plinth", [(2, 0), (3, 1), (4, 0), (1, 1)])
+def test_driver_profiles_and_plinth(tmp_path, profile, plinth):
+    sp = make_species(tmp_path, "01_tst", ["11", "12"], [18.010565, 20.014811], [0.997317, 0.002],
+                      2000, 50 + profile, 995.0, 1015.0)
+    T, P, cut = 1300.0, 0.6, 2.0
+    F.write_param_dat(str(tmp_path / "param.dat"), name="pr", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=1, nbins=NBINS,
+                      profile=profile, removePlinth=plinth)
+    run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    Kd = F.read_out_dat(str(tmp_path / "Out_pr.dat"))[0][:, 1]
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    L = sp["lines"]
+    m = sp["masses"]
+    d = dict(nu=L["nu"], S=L["S"], EL=L["EL"], A=L["A"], delta=L["delta"], vy=L["gself"],
+             n=L["n"], iso=sp["iso"].astype(np.int32), iso_mass=m / NA,
+             iso_ab=sp["abund"] * m / meanMass, iso_sscale=m / meanMass,
+             iso_Q=np.array([q_at(T, Q) for Q in sp["Qtab"]]))
+    st = O.line_params(d, T, P, 0, cut)
+    Ko = O.profile_points(st, NUMIN, DNU, nx, profile, remove_plinth=bool(plinth))
+    e = rel_err(Kd, Ko)
+    print("driver profile %d plinth %d vs oracle %.3e" % (profile, plinth, e))
+    assert e <= 1e-11

@@ -424,3 +424,69 @@ def test_transmission_closed_forms():
     t2 = O.transmission(K2, 1, N, nTr, dTr)[0]
     np.testing.assert_allclose(t2, (23 / 24 + (N - 1 - 1 / 2 + 1 / 6 - 5 / 8) * f) / (N - 1),
                                rtol=1e-13)
+
+
+# ---------------------------------------------------------------- profiles 2-4, plinth
+def _one_line_state(nu, S, ia, y, hw):
+    return dict(nu=np.array([nu]), S=np.array([S]), ialphaD=np.array([ia]), y=np.array([y]),
+                hw=np.array([hw]))
+
+
+def test_profiles_closed_forms():
+    from scipy.special import erf
+    numin, dnu, Nx = 990.0, 0.001, 20001
+    nu, Strue, aD, aL, hw = 1000.0123, 2.5e-20, 0.02, 0.013, 3.0
+    ia = 1.0 / aD
+    st = _one_line_state(nu, Strue * ia, ia, aL * ia, hw)
+    x = (numin + dnu * np.arange(Nx) - nu) * ia
+    inwin = np.abs(numin + dnu * np.arange(Nx) - nu) < hw
+    # binned Gaussian: the erf differences telescope over the window
+    K4 = O.profile_points(st, numin, dnu, Nx, 4)
+    vb = dnu * ia
+    lo, hi = np.nonzero(inwin)[0][[0, -1]]
+    exact = Strue * ia / vb * 0.5 * (erf(x[hi] + vb / 2) - erf(x[lo] - vb / 2))
+    assert abs(K4.sum() / exact - 1) < 1e-13
+    assert abs(K4.sum() * dnu / Strue - 1) < 1e-13          # the window holds the whole Gaussian
+    # Doppler: a sampled Gaussian, normalised
+    K3 = O.profile_points(st, numin, dnu, Nx, 3)
+    assert abs(K3.sum() * dnu / Strue - 1) < 1e-12
+    np.testing.assert_allclose(K3[inwin], Strue * ia / math.sqrt(math.pi) * np.exp(-x[inwin] ** 2),
+                               rtol=1e-15, atol=0)
+    # Lorentz: the integral over the window, (2/pi) atan(hw / alpha_L), sampled
+    K2 = O.profile_points(st, numin, dnu, Nx, 2)
+    expect = Strue * (2 / math.pi) * math.atan(hw / aL)
+    assert abs(K2.sum() * dnu / expect - 1) < 1e-6
+    np.testing.assert_allclose(K2[inwin], Strue / math.pi * aL / ((x[inwin] * aD) ** 2 + aL ** 2),
+                               rtol=1e-13)
+    assert np.all(K2[~inwin] == 0) and np.all(K3[~inwin] == 0) and np.all(K4[~inwin] == 0)
+    # Voigt through the profile path = the Voigt sum
+    K1 = O.profile_points(st, numin, dnu, Nx, 1)
+    np.testing.assert_array_equal(K1, O.voigt_points(st, numin, dnu, Nx))
+
+
+@pytest.mark.parametrize("profile", [1, 2, 3])
+def test_plinth_removal(profile):
+    numin, dnu, Nx = 990.0, 0.002, 10001
+    rng = np.random.default_rng(profile)
+    n = 40
+    nu = np.sort(rng.uniform(995.0, 1005.0, n))
+    ia = 1.0 / rng.uniform(0.005, 0.05, n)
+    st = dict(nu=nu, S=10 ** rng.uniform(-22, -19, n) * ia, ialphaD=ia,
+              y=rng.uniform(0.01, 3.0, n) , hw=rng.uniform(0.05, 2.0, n))
+    K = O.profile_points(st, numin, dnu, Nx, profile)
+    Kp = O.profile_points(st, numin, dnu, Nx, profile, remove_plinth=True)
+    xj = numin + dnu * np.arange(Nx)
+    sub = np.zeros(Nx)
+    for i in range(n):
+        pl = O.profile_value(profile, st["S"][i], st["hw"][i] * ia[i], st["y"][i], dnu * ia[i])
+        cover = np.abs(xj - nu[i]) < st["hw"][i]
+        sub[cover] += pl
+        # the plinth is the profile at the cut: a positive constant below the peak
+        assert 0 <= pl < O.profile_value(profile, st["S"][i], 0.0, st["y"][i], dnu * ia[i])
+    np.testing.assert_allclose(Kp, K - sub, rtol=1e-12, atol=1e-12 * np.max(K))
+    # one line: the plinth-removed profile falls to ~0 at the cut
+    one = {k: v[:1] for k, v in st.items()}
+    K1 = O.profile_points(one, numin, dnu, Nx, profile, remove_plinth=True)
+    cover = np.nonzero(np.abs(xj - nu[0]) < st["hw"][0])[0]
+    edge = K1[cover[[0, -1]]]
+    assert np.all(edge >= -1e-9 * K1.max()) and np.all(edge < 0.05 * K1.max())
