@@ -754,6 +754,66 @@ extern "C" int hk_ktable(const double *K, int nspec, const hk_grid *g, double nu
     return HK_OK;
 }
 
+// Ascending sort of arbitrary bins [seg[b], seg[b+1]) of one spectrum (the
+// per-bin thrust sorts of heliosk.cu:2046-2051 for bins-file and per-bin
+// grids).  Workspace: the segment table and one spectrum for the merge
+// ping-pong.
+extern "C" size_t hk_sort_bins_workspace_bytes(int64_t nx, int nseg) {
+    if (nx <= 0 || nseg <= 0) return 0;
+    return align_up(sizeof(Seg) * (size_t)nseg, 256) + align_up(sizeof(double) * (size_t)nx, 256) +
+           256;
+}
+
+extern "C" int hk_sort_bins(const double *K, int64_t nx, const int64_t *seg, int nseg,
+                            double *Ksorted, void *ws, size_t ws_bytes, hk_stream_t stream) {
+    HK_CHECK_ARG(K && seg && Ksorted && ws && nx > 0 && nseg > 0 && nseg < 65536,
+                 "hk_sort_bins: bad arguments");
+    HK_CHECK_ARG(K != Ksorted, "hk_sort_bins: in-place sort is not supported");
+    HK_CHECK_ARG(seg[0] >= 0 && seg[nseg] <= nx, "hk_sort_bins: segments outside [0, nx)");
+    int64_t maxlen = 0;
+    for (int b = 0; b < nseg; ++b) {
+        HK_CHECK_ARG(seg[b + 1] >= seg[b], "hk_sort_bins: segment %d has negative length", b);
+        maxlen = std::max(maxlen, seg[b + 1] - seg[b]);
+    }
+    const size_t need = hk_sort_bins_workspace_bytes(nx, nseg);
+    if (ws_bytes < need) {
+        set_error("hk_sort_bins: workspace %zu bytes, %zu needed", ws_bytes, need);
+        return HK_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char *base = (char *)align_up((size_t)ws, 256);
+    Seg *segs = (Seg *)base;
+    double *tmp = (double *)(base + align_up(sizeof(Seg) * (size_t)nseg, 256));
+    std::vector<Seg> hs(nseg);
+    for (int b = 0; b < nseg; ++b) { hs[b].start = seg[b]; hs[b].len = seg[b + 1] - seg[b]; }
+    HK_HIP(hipMemcpyAsync(segs, hs.data(), hs.size() * sizeof(Seg), hipMemcpyHostToDevice, s));
+    HK_HIP(hipStreamSynchronize(s));         // hs is host stack memory
+    if (maxlen == 0) return HK_OK;
+    if (maxlen <= kSortLDS) {
+        hipLaunchKernelGGL((bitonic_kernel<kSortLDS>), dim3(1, nseg), dim3(kSortThreads), 0, s, K,
+                           Ksorted, segs, (int64_t)kSortLDS);
+        HK_LAUNCH_CHECK();
+        return HK_OK;
+    }
+    const int64_t chunk = kSortLDS;
+    const int64_t nchunks = (maxlen + chunk - 1) / chunk;
+    int npass = 0;
+    for (int64_t w = chunk; w < maxlen; w *= 2) ++npass;
+    double *first = (npass % 2 == 0) ? Ksorted : tmp;
+    double *other = (npass % 2 == 0) ? tmp : Ksorted;
+    hipLaunchKernelGGL((bitonic_kernel<kSortLDS>), dim3((unsigned)nchunks, nseg),
+                       dim3(kSortThreads), 0, s, K, first, segs, chunk);
+    HK_LAUNCH_CHECK();
+    double *src = first, *dst = other;
+    for (int64_t w = chunk; w < maxlen; w *= 2) {
+        hipLaunchKernelGGL(merge_lds_kernel, dim3((unsigned)((maxlen + kMB - 1) / kMB), nseg),
+                           dim3(256), 0, s, src, dst, segs, w);
+        HK_LAUNCH_CHECK();
+        std::swap(src, dst);
+    }
+    return HK_OK;
+}
+
 extern "C" int hk_ktable_reconstruct(const double *coeff, const int32_t *nxmin, int nspec,
                                      int nbins, int Nxb, int nC, double kmin, double unit_scale,
                                      double *Ky, double *scratch, hk_stream_t stream) {

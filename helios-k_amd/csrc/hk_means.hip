@@ -68,6 +68,37 @@ __global__ __launch_bounds__(kMeanThreads) void means_partial_kernel(
     if (t < 4) part[(size_t)blockIdx.x * 4 + t] = s[t][0];
 }
 
+// Trapezoid form on explicit points x (IntegrateMean_kernel with
+// useIndividualX, resample.h:700-716): per interval [x_j, x_j+1] the mean of
+// the end values times the width; the Rosseland term needs both K != 0.
+__global__ __launch_bounds__(kMeanThreads) void means_x_partial_kernel(
+    const double *__restrict__ K, const double *__restrict__ x, int64_t nx, double T,
+    double *__restrict__ part) {
+    __shared__ double s[4][kMeanThreads];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kMeanThreads;
+    for (int64_t j = (int64_t)blockIdx.x * kMeanThreads + threadIdx.x; j + 1 < nx; j += stride) {
+        double B0, dB0, B1, dB1;
+        const double x0 = x[j], x1 = x[j + 1];
+        planck_terms(x0, T, B0, dB0);
+        planck_terms(x1, T, B1, dB1);
+        const double k0 = K[j], k1 = K[j + 1], w = x1 - x0;
+        a0 += 0.5 * (B0 * k0 + B1 * k1) * w;
+        if (k0 != 0.0 && k1 != 0.0) a1 += 0.5 * (dB0 / k0 + dB1 / k1) * w;
+        a2 += 0.5 * (B0 + B1) * w;
+        a3 += 0.5 * (dB0 + dB1) * w;
+    }
+    const int t = threadIdx.x;
+    s[0][t] = a0; s[1][t] = a1; s[2][t] = a2; s[3][t] = a3;
+    __syncthreads();
+    for (int h = kMeanThreads / 2; h > 0; h >>= 1) {
+        if (t < h)
+            for (int q = 0; q < 4; ++q) s[q][t] += s[q][t + h];
+        __syncthreads();
+    }
+    if (t < 4) part[(size_t)blockIdx.x * 4 + t] = s[t][0];
+}
+
 __global__ __launch_bounds__(kMeanThreads) void means_final_kernel(
     const double *__restrict__ part, int nblocks, double *__restrict__ sums) {
     __shared__ double s[4][kMeanThreads];
@@ -157,6 +188,29 @@ extern "C" int hk_means(const double *K, const hk_grid *grid, double T, double *
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(means_partial_kernel, dim3(nblocks), dim3(kMeanThreads), 0, s, K,
                        grid->numin, grid->dnu, grid->nx, T, part);
+    HK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(means_final_kernel, dim3(1), dim3(kMeanThreads), 0, s, part, nblocks,
+                       sums);
+    HK_LAUNCH_CHECK();
+    return HK_OK;
+}
+
+extern "C" int hk_means_x(const double *K, const double *x, int64_t nx, double T, double *sums,
+                          void *workspace, size_t workspace_bytes, hk_stream_t stream) {
+    HK_CHECK_ARG(K && x && sums && workspace, "hk_means_x: null argument");
+    HK_CHECK_ARG(nx > 0, "hk_means_x: no points");
+    HK_CHECK_ARG(T > 0.0, "hk_means_x: T must be > 0");
+    if (workspace_bytes < hk_means_workspace_bytes()) {
+        hk::set_error("hk_means_x: workspace %zu bytes, %zu needed", workspace_bytes,
+                      hk_means_workspace_bytes());
+        return HK_ERR_WORKSPACE;
+    }
+    double *part = (double *)hk::align_up((size_t)workspace, 256);
+    const int64_t want = (nx + kMeanThreads - 1) / kMeanThreads;
+    const int nblocks = (int)(want < kMeanBlocks ? want : kMeanBlocks);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(means_x_partial_kernel, dim3(nblocks), dim3(kMeanThreads), 0, s, K, x, nx, T,
+                       part);
     HK_LAUNCH_CHECK();
     hipLaunchKernelGGL(means_final_kernel, dim3(1), dim3(kMeanThreads), 0, s, part, nblocks,
                        sums);

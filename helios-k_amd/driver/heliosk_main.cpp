@@ -28,8 +28,8 @@
 // (OutputEdgesFile).
 //
 // Options the reference has that this driver does not implement are
-// rejected with an error message: CIA, bins files, "Nnu per bin" and
-// sub-Lorentzian wings.  Profiles 1-4 and removePlinth are supported.
+// rejected with an error message: CIA and sub-Lorentzian wings.  Profiles
+// 1-4, removePlinth, bins files and "Nnu per bin" grids are supported.
 //
 // Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
@@ -663,8 +663,10 @@ bool init_K(const Param &p, const hk_grid &grid, size_t nP, double *K_d, hipStre
 // device (hk_lines_sort) and summed at every pressure; the host reads the
 // next chunk into the other buffer while the device works on this one.
 bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q, double Sscale,
-                 double meanMass, const hk_grid &grid, const std::vector<double> &Plist,
-                 double *K_d, hipStream_t s, long long maxlines, Timing &tm) {
+                 double meanMass, const std::vector<hk_grid> &pieces,
+                 const std::vector<int64_t> &piece_off, int64_t Nx,
+                 const std::vector<double> &Plist, double *K_d, hipStream_t s, long long maxlines,
+                 Timing &tm) {
     // file range (heliosk.cu:939-957)
     int fi0 = m.nFiles, fi1 = 0;
     const double cut = p.cut == 0.0 ? 1.0e30 : p.cut;
@@ -700,7 +702,8 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
     const size_t stb = hk_line_state_bytes(cap);
     const size_t sortb = hk_lines_sort_workspace_bytes(cap);
     if (sortb == 0) return fail("Error: %s", hk_last_error());
-    const size_t wsb = hk_profile_workspace_bytes(&grid, cap);
+    size_t wsb = 0;
+    for (const hk_grid &g : pieces) wsb = std::max(wsb, hk_profile_workspace_bytes(&g, cap));
     if (!dst.alloc(stb) || !dsws.alloc(sortb) || !dws.alloc(wsb))
         return fail("Error: device allocation of the line state and workspaces failed");
 
@@ -783,8 +786,9 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
             HKCHK(hk_line_params(&srt, &tp, &c, &st, s));
             mk.voi.push_back(ev.size());
             if (!event(e)) { ok = false; break; }
-            HKCHK(hk_profile_accumulate(&st, &grid, p.kmin, 1, p.profile, plinth, mode,
-                                        K_d + iP * grid.nx, dws.p, wsb, s));
+            for (size_t q = 0; q < pieces.size(); ++q)
+                HKCHK(hk_profile_accumulate(&st, &pieces[q], p.kmin, 1, p.profile, plinth, mode,
+                                            K_d + iP * Nx + piece_off[q], dws.p, wsb, s));
             mk.end.push_back(ev.size());
             if (!event(e)) { ok = false; break; }
         }
@@ -853,6 +857,7 @@ bool write_info(FILE *f, const Param &p, int64_t Nx, const char *device) {
     std::fprintf(f, "pathToK = %s\n", p.pathK.c_str());
     std::fprintf(f, "dostoreK = %d\n", p.doStoreK);
     std::fprintf(f, "nbins = %d\n", p.nbins);
+    if (p.bins != "-") std::fprintf(f, "use Individual bins: %s\n", p.bins.c_str());
     std::fprintf(f, "kmin = %g\n", p.kmin);
     std::fprintf(f, "qalphaL = %g\n", p.qalphaL);
     std::fprintf(f, "gammaF = %g\n", p.gammaF);
@@ -905,6 +910,55 @@ void write_bin_K(FILE *o, const Param &p, const double *K, int64_t Nxb, double u
     }
 }
 
+// K(y) of irregular bins (the useIndividualBins branch of heliosk.cu:
+// 2292-2345): bin b holds the points [seg[b], seg[b+1]), y = j/(N_b - 1);
+// with output edges the running interval sum carries on across bins and
+// only the edge index restarts with each bin, as there.
+bool write_bins_file_K(FILE *kb, const std::string &kname, const Param &p, const double *K,
+                       const std::vector<int64_t> &seg, double unitScale,
+                       const std::vector<double> &edges, double P) {
+    const int ne = (int)edges.size();
+    int iedge = 0, nedge = 0;
+    double sedge = 0.0;
+    for (int b = 0; b < p.nbins; ++b) {
+        FILE *o = kb;
+        if (p.doStoreK == 2) {
+            char nm[1200];
+            std::snprintf(nm, sizeof(nm), "%s%04d.dat", kname.c_str(), b);
+            o = std::fopen(nm, "a");
+        }
+        if (!o) return fail("Error: cannot open %s output", kname.c_str());
+        const int64_t N = seg[b + 1] - seg[b];
+        for (int64_t j = 0; j < N; ++j) {
+            const double y = j / (double)(N - 1);
+            const double y1 = (j + 1) / (double)(N - 1);
+            const double k = K[seg[b] + j] * unitScale;
+            if (ne == 0) {
+                if (p.nP == 1) std::fprintf(o, "%g %.20g\n", y, k);
+                else std::fprintf(o, "%g %.20g %.20g %.20g %lld\n", y, k, p.T, P, (long long)j);
+                continue;
+            }
+            ++nedge;
+            sedge += k;
+            if (iedge < ne && y <= edges[iedge] && edges[iedge] <= y1) {
+                if (iedge > 0) {
+                    const double mid = 0.5 * (edges[iedge] + edges[iedge - 1]);
+                    if (p.nP == 1) std::fprintf(o, "%g %.20g\n", mid, sedge / (double)nedge);
+                    else std::fprintf(o, "%g %.20g %.20g %.20g %d\n", mid, sedge / (double)nedge,
+                                      p.T, P, iedge - 1);
+                }
+                ++iedge;
+                nedge = 0;
+                sedge = 0.0;
+            }
+        }
+        iedge = 0;
+        if (p.doStoreK == 1) std::fprintf(o, "\n\n");
+        else std::fclose(o);
+    }
+    return true;
+}
+
 bool run(int argc, char **argv) {
     Param p;
     if (!read_parameters(p, "param.dat", argc, argv)) return false;
@@ -914,10 +968,7 @@ bool run(int argc, char **argv) {
         return fail("Error, remove plinth is not supported for profile 4");   // heliosk.cu:410-413
     if (p.subLorentzian != "-") return fail("Error: sub-Lorentzian files are not supported");
     if (p.ciaSystem != "-") return fail("Error: CIA systems are not supported");
-    if (p.bins != "-") return fail("Error: bins files are not supported");
-    if (p.Nxb != 0) return fail("Error: 'Nnu per bin' is not supported");
     if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
-    if (p.nbins <= 0) return fail("Error: nbins must be > 0");
     if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
         return fail("Error: maxlines must lie in 1 .. 2147483647");
     if (!(p.dnu > 0.0)) return fail("Error: dnu must be > 0");
@@ -952,16 +1003,92 @@ bool run(int argc, char **argv) {
         p.nSpecies = (int)spNames.size();
     }
 
-    const double span = (p.numax - p.numin) / p.dnu + 0.5;
-    if (span >= 2147483647.0) return fail("Error: Nx too large, integer overflow. %g", span);
-    const int64_t Nx = (int64_t)span;
-    if (Nx <= 0) return fail("Error: empty grid");
-    if (Nx % p.nbins != 0) {
-        std::printf("Error: range cannot be divided evenly in bins. %lld %d %g\n", (long long)Nx,
-                    p.nbins, Nx / (double)p.nbins);
-        return false;
+    // bins: equal in wavenumber, or from a bins file (host.h:388-411,
+    // readBinFile host.h:522-538, heliosk.cu:417-441); points: uniform dnu, or
+    // "Nnu per bin" points spread evenly over each bin (heliosk.cu:406-408,
+    // 448-476, setX_kernel voigt.h:751-767)
+    const bool useBins = p.bins != "-";
+    const bool useX = p.Nxb != 0;
+    std::vector<double> bnd;
+    if (useBins) {
+        std::printf("Use bins in file %s\n", p.bins.c_str());
+        int nb = 0;
+        if (!count_values(p.bins, nb, "bins")) return false;
+        if (!read_values(p.bins, bnd) || nb < 2) return fail("Error: bins file too short: %s", p.bins.c_str());
+        p.nbins = nb - 1;
+        for (int i = 0; i <= p.nbins; ++i)
+            if (!(bnd[i] > (i ? bnd[i - 1] : -100000.0)))
+                return fail("Error; bin boundaries not monotonic growing");
+        bnd.resize(p.nbins + 1);
+        p.numin = bnd[0];
+        p.numax = bnd[p.nbins];
+        if (p.doResampling > 0)
+            return fail("Error: The resampling function is not supported for the bin-file option");
+        if (p.doTransmission > 0)
+            return fail("Error: The transmission function is not supported for the bin-file option");
+    } else {
+        if (p.nbins <= 0) return fail("Error: nbins must be > 0");
+        bnd.resize(p.nbins + 1);
+        for (int i = 0; i < p.nbins; ++i) bnd[i] = p.numin + i * (p.numax - p.numin) / ((double)p.nbins);
+        bnd[p.nbins] = p.numax;
     }
-    const int Nxb = (int)(Nx / p.nbins);
+    int64_t Nx = 0;
+    if (!useX) {
+        const double span = (p.numax - p.numin) / p.dnu + 0.5;
+        if (span >= 2147483647.0) return fail("Error: Nx too large, integer overflow. %g", span);
+        Nx = (int64_t)span;
+        if (Nx <= 0) return fail("Error: empty grid");
+        if (Nx % p.nbins != 0) {
+            std::printf("Error: range cannot be divided evenly in bins. %lld %d %g\n", (long long)Nx,
+                        p.nbins, Nx / (double)p.nbins);
+            return false;
+        }
+        p.Nxb = (int)(Nx / p.nbins);
+    } else {
+        if ((double)p.nbins * (double)p.Nxb >= 2147483647.0 || p.Nxb < 0)
+            return fail("Error: Nx too large, integer overflow. %g", (double)p.nbins * p.Nxb);
+        Nx = (int64_t)p.nbins * p.Nxb;
+        if (p.doResampling > 0)
+            return fail("Error: The resampling function is not supported for unequal spacing option");
+        if (p.doTransmission > 0)
+            return fail("Error: The transmission function is not supported for unequal spacing option");
+    }
+    const int Nxb = p.Nxb;
+    // the grid as uniform pieces: one, or one per bin; x of every point
+    std::vector<hk_grid> pieces;
+    std::vector<int64_t> piece_off;
+    std::vector<double> x_h(Nx);
+    if (!useX) {
+        pieces.push_back(hk_grid{p.numin, p.dnu, Nx});
+        piece_off.push_back(0);
+        for (int64_t j = 0; j < Nx; ++j) x_h[j] = std::fma((double)j, p.dnu, p.numin);
+    } else {
+        for (int b = 0; b < p.nbins; ++b) {
+            const double dnub = (bnd[b + 1] - bnd[b]) / ((double)Nxb);
+            pieces.push_back(hk_grid{bnd[b], dnub, (int64_t)Nxb});
+            piece_off.push_back((int64_t)b * Nxb);
+            for (int k = 0; k < Nxb; ++k) x_h[(int64_t)b * Nxb + k] = std::fma((double)k, dnub, bnd[b]);
+        }
+    }
+    // first point of each bin by the binKey rule (voigt.h:779-835): bin b
+    // holds the x with bnd[b] <= x < bnd[b+1]; x >= numax fall in bin nbins
+    std::vector<int64_t> seg(p.nbins + 2);
+    {
+        auto key = [&](int64_t j) {
+            const double x = x_h[j];
+            if (x >= p.numax) return p.nbins;
+            const int64_t u = std::upper_bound(bnd.begin(), bnd.end(), x) - bnd.begin() - 1;
+            return (u >= 0 && u < p.nbins) ? (int)u : 0;
+        };
+        for (int b = 0; b <= p.nbins + 1; ++b) {
+            int64_t lo = 0, hi = Nx;
+            while (lo < hi) {
+                const int64_t m = (lo + hi) / 2;
+                if (key(m) < b) lo = m + 1; else hi = m;
+            }
+            seg[b] = lo;
+        }
+    }
     if (Nxb < p.nC) {
         std::printf("Number of points per bin smaller than the number of Chebyshev coefficients: "
                     "Changed nC to %d\n", Nxb);
@@ -1002,26 +1129,25 @@ bool run(int argc, char **argv) {
         std::printf("kmin %g\n", p.kmin);
     }
 
-    hk_grid grid = {p.numin, p.dnu, Nx};
+    const hk_grid whole = {0.0, 1.0, Nx};     // index space of all Nx points
     hipStream_t s;
     HIPCHK(hipStreamCreate(&s));
     DevBuf Kd;
     if (!Kd.alloc((size_t)p.nP * Nx * sizeof(double))) return fail("Error: K allocation failed");
     double *K_d = Kd.as<double>();
-    std::vector<double> K_h(Nx), x_h(Nx);
-    for (int64_t j = 0; j < Nx; ++j) x_h[j] = std::fma((double)j, p.dnu, p.numin);
+    std::vector<double> K_h(Nx);
 
     Timing tm;
     auto t_all = std::chrono::steady_clock::now();
     if (p.doStoreFullK >= 0) {
-        if (!init_K(p, grid, Plist.size(), K_d, s)) { std::fclose(info); return false; }
+        if (!init_K(p, whole, Plist.size(), K_d, s)) { std::fclose(info); return false; }
         for (size_t k = 0; k < mols.size(); ++k) {
             std::vector<double> Q;
             if (!read_partition(mols[k], p.T, Q)) { std::fclose(info); return false; }
             const double Sscale = p.nSpecies > 1 ? spAb[k] : 1.0;
             std::printf("mean mass %g, Sscale %g\n", meanMass, Sscale);
-            if (!add_species(p, mols[k], Q, Sscale, meanMass, grid, Plist, K_d, s, p.maxlines,
-                             tm)) {
+            if (!add_species(p, mols[k], Q, Sscale, meanMass, pieces, piece_off, Nx, Plist, K_d,
+                             s, p.maxlines, tm)) {
                 std::fclose(info);
                 return false;
             }
@@ -1055,7 +1181,7 @@ bool run(int argc, char **argv) {
                 int er = p.nP == 1 ? std::fscanf(o, "%lf %lf", &x, &k) == 2
                                    : std::fscanf(o, "%lf %lf %lf %lf", &x, &k, &t, &pp) == 4;
                 if (!er) { std::fclose(o); return fail("Error: Input file too short %s", fn.c_str()); }
-                x_h[j] = x;
+                (void)x;
                 K_h[j] = k / unitScale;
             }
             HIPCHK(hipMemcpy(K_d + (size_t)iP * Nx, K_h.data(), Nx * sizeof(double), hipMemcpyHostToDevice));
@@ -1104,9 +1230,19 @@ bool run(int argc, char **argv) {
         const double T = p.T;
         const double integral1 = sigma * T * T * T * T / M_PI;
         const double integral2 = M_PI / (4.0 * sigma * T * T * T);
+        DevBuf xd;
+        if (useX) {
+            if (!xd.alloc(Nx * sizeof(double))) return fail("Error: allocation for the means failed");
+            HIPCHK(hipMemcpy(xd.p, x_h.data(), Nx * sizeof(double), hipMemcpyHostToDevice));
+        }
         for (int iP = 0; iP < p.nP; ++iP) {
             double m[4];
-            HKCHK(hk_means(K_d + (size_t)iP * Nx, &grid, T, msum.as<double>(), mws.p, mwb, s));
+            if (useX)   // trapezoid rule on the per-bin points (resample.h:700-716)
+                HKCHK(hk_means_x(K_d + (size_t)iP * Nx, xd.as<double>(), Nx, T, msum.as<double>(),
+                                 mws.p, mwb, s));
+            else
+                HKCHK(hk_means(K_d + (size_t)iP * Nx, &pieces[0], T, msum.as<double>(), mws.p,
+                               mwb, s));
             HIPCHK(hipMemcpyAsync(m, msum.p, sizeof(m), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (p.nP == 1) {
@@ -1139,13 +1275,55 @@ bool run(int argc, char **argv) {
     DevBuf ws, coeff, nxmin, Ks, Ky, scr;
     std::vector<double> coeff_h;
     std::vector<int32_t> nxmin_h;
-    if (p.doResampling > 0 || p.doStoreK > 0 || p.doTransmission > 0) {
+    if ((useBins || useX) && p.doStoreK > 0) {
+        // per-bin sort and K(y) of irregular bins or per-bin grids
+        const size_t sb = hk_sort_bins_workspace_bytes(Nx, p.nbins + 1);
+        if (!ws.alloc(sb) || !Ks.alloc(Nx * sizeof(double)))
+            return fail("Error: allocation for the bin sort failed");
+        const std::string kname = "Out_" + p.name + "_bin";
+        FILE *kb = nullptr;
+        if (p.doStoreK == 1) kb = std::fopen((kname + ".dat").c_str(), fmode);
+        if (p.doStoreK == 2 && p.replaceFiles == 1) {
+            for (int b = 0; b < p.nbins; ++b) {
+                char nm[1200];
+                std::snprintf(nm, sizeof(nm), "%s%04d.dat", kname.c_str(), b);
+                FILE *t = std::fopen(nm, "w");
+                if (t) std::fclose(t);
+            }
+        }
+        for (int iP = 0; iP < p.nP; ++iP) {
+            HKCHK(hk_sort_bins(K_d + (size_t)iP * Nx, Nx, seg.data(), p.nbins + 1, Ks.as<double>(),
+                               ws.p, sb, s));
+            HIPCHK(hipMemcpyAsync(K_h.data(), Ks.p, Nx * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (!useBins) {
+                for (int b = 0; b < p.nbins; ++b) {
+                    FILE *o = kb;
+                    if (p.doStoreK == 2) {
+                        char nm[1200];
+                        std::snprintf(nm, sizeof(nm), "%s%04d.dat", kname.c_str(), b);
+                        o = std::fopen(nm, "a");
+                    }
+                    if (!o) return fail("Error: cannot open %s output", kname.c_str());
+                    write_bin_K(o, p, K_h.data() + (int64_t)b * Nxb, Nxb, unitScale, edges,
+                                Plist[iP]);
+                    if (p.doStoreK == 1) std::fprintf(o, "\n\n");
+                    else std::fclose(o);
+                }
+            } else if (!write_bins_file_K(kb, kname, p, K_h.data(), seg, unitScale, edges,
+                                          Plist[iP])) {
+                return false;
+            }
+        }
+        if (kb) std::fclose(kb);
+    } else if (!useBins && !useX &&
+               (p.doResampling > 0 || p.doStoreK > 0 || p.doTransmission > 0)) {
         const int nC = std::max(p.nC, 1);
         const size_t wsb = hk_ktable_workspace_bytes(Nx, p.nbins, nC, 1);
         if (!ws.alloc(wsb) || !coeff.alloc((size_t)p.nbins * nC * sizeof(double)) ||
             !nxmin.alloc((size_t)p.nbins * sizeof(int32_t)) || !Ks.alloc(Nx * sizeof(double)))
             return fail("Error: k-table allocation failed");
-        HKCHK(hk_ktable_prepare(&grid, p.numax, p.nbins, nC, ws.p, wsb));
+        HKCHK(hk_ktable_prepare(&pieces[0], p.numax, p.nbins, nC, ws.p, wsb));
         if (need_recon)
             if (!Ky.alloc(Nx * sizeof(double)) || !scr.alloc(Nx * sizeof(double)))
                 return fail("Error: k-table allocation failed");
@@ -1173,7 +1351,7 @@ bool run(int argc, char **argv) {
         nxmin_h.resize(p.nbins);
         for (int iP = 0; iP < p.nP; ++iP) {
             double *KP = K_d + (size_t)iP * Nx;
-            HKCHK(hk_ktable(KP, 1, &grid, p.numax, p.nbins, nC, p.kmin, unitScale,
+            HKCHK(hk_ktable(KP, 1, &pieces[0], p.numax, p.nbins, nC, p.kmin, unitScale,
                             coeff.as<double>(), nxmin.as<int32_t>(), Ks.as<double>(), ws.p, wsb,
                             s));
             HIPCHK(hipMemcpyAsync(coeff_h.data(), coeff.p, coeff_h.size() * sizeof(double),

@@ -93,6 +93,11 @@ def lib():
         L.hk_profile_workspace_bytes.argtypes = [P(Grid), c_i64]
         L.hk_profile_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, c_int, c_int,
                                             c_vp, c_vp, c_size, c_vp]
+        L.hk_means_x.argtypes = [c_vp, c_vp, c_i64, c_d, c_vp, c_vp, c_size, c_vp]
+        L.hk_sort_bins_workspace_bytes.restype = c_size
+        L.hk_sort_bins_workspace_bytes.argtypes = [c_i64, c_int]
+        L.hk_sort_bins.argtypes = [c_vp, c_i64, ctypes.POINTER(c_i64), c_int, c_vp, c_vp, c_size,
+                                   c_vp]
         L.hk_means_workspace_bytes.restype = c_size
         L.hk_means_workspace_bytes.argtypes = []
         L.hk_means.argtypes = [c_vp, P(Grid), c_d, c_vp, c_vp, c_size, c_vp]
@@ -138,6 +143,7 @@ def check(rc):
 EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state_init",
             "hk_lines_sort_workspace_bytes", "hk_lines_sort", "hk_means_workspace_bytes",
             "hk_means", "hk_transmission", "hk_profile_workspace_bytes", "hk_profile_accumulate",
+            "hk_means_x", "hk_sort_bins_workspace_bytes", "hk_sort_bins",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
             "hk_voigt_autotune",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",

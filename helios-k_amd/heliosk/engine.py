@@ -22,7 +22,7 @@ NA = 6.0221412927e23          # define.h:42
 __all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
            "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
-           "ktable_reconstruct", "adjoint", "synth_lines", "means", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
+           "ktable_reconstruct", "adjoint", "synth_lines", "means", "means_x", "sort_bins", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
 
 
 def _dev():
@@ -402,6 +402,30 @@ def means(K, grid, T, stream=None):
     check(lib().hk_means(_ptr(K), ctypes.byref(grid), float(T), _ptr(sums), _ptr(ws), nbytes,
                          _stream(stream)))
     return sums
+
+
+def means_x(K, x, T, stream=None):
+    """Mean sums on explicit points x (trapezoid rule)."""
+    K = K.contiguous(); x = x.contiguous()
+    sums = torch.empty(4, dtype=torch.float64, device=K.device)
+    nbytes = lib().hk_means_workspace_bytes()
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
+    check(lib().hk_means_x(_ptr(K), _ptr(x), K.numel(), float(T), _ptr(sums), _ptr(ws), nbytes,
+                           _stream(stream)))
+    return sums
+
+
+def sort_bins(K, seg, stream=None):
+    """K with each bin [seg[b], seg[b+1]) sorted ascending."""
+    K = K.contiguous()
+    seg = np.ascontiguousarray(seg, dtype=np.int64)
+    out = torch.empty_like(K)
+    nseg = seg.size - 1
+    nbytes = lib().hk_sort_bins_workspace_bytes(K.numel(), nseg)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
+    check(lib().hk_sort_bins(_ptr(K), K.numel(), seg.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                             nseg, _ptr(out), _ptr(ws), nbytes, _stream(stream)))
+    return out
 
 
 def transmission(Ks, nbins, Nxb, nTr, dTr, kmin=0.0, nxmin=None, stream=None):

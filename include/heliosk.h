@@ -287,6 +287,13 @@ size_t hk_means_workspace_bytes(void);
 int hk_means(const double *K, const hk_grid *grid, double T, double *sums,
              void *workspace, size_t workspace_bytes, hk_stream_t stream);
 
+/* The same sums on explicit points x (per-bin grids: the reference's
+ * useIndividualX branch, resample.h:700-716): trapezoid rule,
+ * sum_j (f_j + f_j+1)/2 (x_j+1 - x_j); the Rosseland term only where both
+ * K are nonzero. */
+int hk_means_x(const double *K, const double *x, int64_t nx, double T, double *sums,
+               void *workspace, size_t workspace_bytes, hk_stream_t stream);
+
 /* Transmission function of each bin (Integrate_kernel, resample.h:541-614;
  * heliosk.cu:2400-2440).  Ksorted holds nbins bins of Nxb ascending values
  * (hk_ktable's Ksorted, or its reconstruction); nxmin (device, NULL = all 0)
@@ -305,6 +312,14 @@ int hk_voigt_adjoint(const hk_lines *lines, const hk_tp *tp, const hk_cut *cut,
                      const hk_grid *grid, const double *gK, double *gS,
                      double *gvy, double *gTP, void *workspace,
                      size_t workspace_bytes, hk_stream_t stream);
+
+/* Ascending sort of the bins [seg[b], seg[b+1]), b < nseg, of one
+ * spectrum of nx points into Ksorted (the per-bin thrust sorts of
+ * heliosk.cu:2046-2051 for irregular bins).  seg is a host array of nseg+1
+ * indices; the call returns after the segment table is uploaded. */
+size_t hk_sort_bins_workspace_bytes(int64_t nx, int nseg);
+int hk_sort_bins(const double *K, int64_t nx, const int64_t *seg, int nseg, double *Ksorted,
+                 void *workspace, size_t workspace_bytes, hk_stream_t stream);
 
 /* Host helpers. */
 /* First grid index of each bin by binKey semantics (voigt.h:779-806):

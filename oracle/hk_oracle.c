@@ -58,6 +58,10 @@
  *      1953-2000 before SimpsonCoefficient at 2392).  Pinned by
  *      plots/p009/Out_p{1,3}_mean.dat and by closed forms (constant K gives
  *      both means = K; sum B dnu -> sigma T^4 / pi).
+ *  hko_means_x
+ *      The trapezoid branch of IntegrateMean_kernel (resample.h:700-716) for
+ *      per-bin grids.  Pinned by closed forms (constant K; the trapezoid
+ *      integral of B converges to sigma T^4 / pi).
  *  hko_transmission
  *      Integrate_kernel (resample.h:541-614) with SimpsonCoefficient's end
  *      weights (resample.h:618-626).  Pinned by plots/p006/Out_i_tr0000.dat
@@ -736,6 +740,31 @@ void hko_means(const double *K, double numin, double dnu, int64_t Nx, double T,
         if (K[j] != 0.0) hko_nsum(&s[1], &cmp[1], dB / K[j]);
         hko_nsum(&s[2], &cmp[2], B);
         hko_nsum(&s[3], &cmp[3], dB);
+    }
+    for (int q = 0; q < 4; ++q) sums[q] = s[q] + cmp[q];
+}
+
+/* resample.h:700-716 (IntegrateMean_kernel, useIndividualX = 1): trapezoid
+ * rule on explicit points. */
+void hko_means_x(const double *K, const double *x, int64_t Nx, double T, double *sums) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, cmp[4] = {0.0, 0.0, 0.0, 0.0};
+    double Bp = 0.0, dBp = 0.0;
+    for (int64_t j = 0; j < Nx; ++j) {
+        double xx = x[j];
+        double t1 = 2.0 * HKO_H * xx * xx * xx * HKO_C * HKO_C;
+        double t2 = HKO_H * xx * HKO_C / (HKO_KB * T);
+        double e = exp(t2);
+        double B = t1 / (e - 1.0);
+        double dB = t1 * t2 / (T * (e - 2.0 + 1.0 / e));
+        if (xx == 0.0) { B = 0.0; dB = 0.0; }
+        if (j > 0) {
+            double w = x[j] - x[j - 1], k0 = K[j - 1], k1 = K[j];
+            hko_nsum(&s[0], &cmp[0], 0.5 * (Bp * k0 + B * k1) * w);
+            if (k0 != 0.0 && k1 != 0.0) hko_nsum(&s[1], &cmp[1], 0.5 * (dBp / k0 + dB / k1) * w);
+            hko_nsum(&s[2], &cmp[2], 0.5 * (Bp + B) * w);
+            hko_nsum(&s[3], &cmp[3], 0.5 * (dBp + dB) * w);
+        }
+        Bp = B; dBp = dB;
     }
     for (int q = 0; q < 4; ++q) sums[q] = s[q] + cmp[q];
 }

@@ -95,6 +95,8 @@ void hko_reconstruct(const double *coeff, const int *Nxmin, int Nxb,
  * x_j = fma(j, dnu, numin) (resample.h:635-771, plain sums). */
 void hko_means(const double *K, double numin, double dnu, int64_t Nx, double T,
                double *sums);
+/* The same sums on explicit points x, trapezoid rule (resample.h:700-716). */
+void hko_means_x(const double *K, const double *x, int64_t Nx, double T, double *sums);
 /* Transmission of each of nbins sorted bins of Nxb values
  * (resample.h:541-614); nxmin may be NULL (all 0). */
 void hko_transmission(const double *Ks, int nbins, int Nxb, const int32_t *nxmin,

@@ -56,6 +56,7 @@ def lib():
         L.hko_profile_value.argtypes = [c_int, c_d, c_d, c_d, c_d]
         L.hko_profile_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d, c_int,
                                                               c_int, _D, c_int])
+        L.hko_means_x.argtypes = [_D, _D, c_i64, c_d, _D]
         L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
         _lib = L
     return _lib
@@ -245,3 +246,12 @@ def transmission(Ks, nbins, Nxb, nTr, dTr, kmin=0.0, nxmin=None):
     lib().hko_transmission(_p(Ks), int(nbins), int(Nxb), None if nx is None else _p(nx, _I32),
                            float(kmin), int(nTr), float(dTr), _p(Tr))
     return Tr.reshape(nbins, nTr)
+
+
+def means_x(K, x, T):
+    """Trapezoid mean sums on explicit points x."""
+    K = _f64(K); x = _f64(x)
+    assert K.size == x.size
+    out = np.empty(4)
+    lib().hko_means_x(_p(K), _p(x), int(K.size), float(T), _p(out))
+    return out

@@ -68,7 +68,6 @@ def _setup(tmp_path, **kw):
     (dict(profile=4, removePlinth=1), "Error, remove plinth is not supported for profile 4"),
     (dict(numin=0.0, numax=10.0, dnu=0.01, nbins=7), "Error: range cannot be divided evenly in bins."),
     (dict(ciaSystem="H2-H2"), "Error: CIA systems are not supported"),
-    (dict(binsFile="bins.dat"), "Error: bins files are not supported"),
 ])
 def test_driver_rejects_like_the_reference(tmp_path, kw, msg):
     _setup(tmp_path, **kw)
@@ -111,6 +110,31 @@ def test_edge_means_hand_case():
     rows = edge_means(np.linspace(0, 1, 11), K, [0.0, 0.25, 0.5, 1.0])
     assert rows == [(0.125, (1 + 4) / 2), (0.375, (9 + 16) / 2),
                     (0.75, (25 + 36 + 49 + 64 + 81) / 5)]
+
+
+def test_driver_bins_and_per_bin_points_errors(tmp_path):
+    (tmp_path / "bins.txt").write_text("1000\n1001.5\n1004\n1010\n")
+    for kw, msg in [
+        (dict(binsFile="bins.txt", doResampling=1),
+         "Error: The resampling function is not supported for the bin-file option"),
+        (dict(binsFile="bins.txt", doTransmission=1),
+         "Error: The transmission function is not supported for the bin-file option"),
+        (dict(Nnu_per_bin=100, doResampling=1),
+         "Error: The resampling function is not supported for unequal spacing option"),
+        (dict(Nnu_per_bin=100, doTransmission=2),
+         "Error: The transmission function is not supported for unequal spacing option"),
+        (dict(binsFile="missing.txt"), "Error: bins file not found: missing.txt"),
+        (dict(binsFile="bins.txt", dnu=0.001),      # 10000 points, 3 bins
+         "Error: range cannot be divided evenly in bins. 10000 3"),
+    ]:
+        _setup(tmp_path, **kw)
+        rc, out = run_driver(str(tmp_path))
+        assert msg in out, (kw, out)
+    (tmp_path / "bins.txt").write_text("1000\n1004\n1002\n1010\n")
+    _setup(tmp_path, binsFile="bins.txt")
+    rc, out = run_driver(str(tmp_path))
+    assert "Use bins in file bins.txt" in out
+    assert "Error; bin boundaries not monotonic growing" in out, out
 
 
 def test_driver_undefined_key(tmp_path):

@@ -422,3 +422,67 @@ def test_driver_profiles_and_plinth(tmp_path, profile, plinth):
     e = rel_err(Kd, Ko)
     print("driver profile %d plinth %d vs oracle %.3e" % (profile, plinth, e))
     assert e <= 1e-11
+
+
+def test_driver_points_per_bin(tmp_path):
+    """"Nnu per bin": each of the equal bins gets its own uniform points
+    (setX_kernel, voigt.h:751-767); K, the trapezoid means and the per-bin
+    K(y) against the oracle summed on each bin's grid."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 2000, 61, 995.0, 1015.0)
+    T, P, cut, nbins, Nxb = 1400.0, 0.9, 5.0, 5, 1733
+    F.write_param_dat(str(tmp_path / "param.dat"), name="x", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=1, nbins=nbins,
+                      Nnu_per_bin=Nxb, doMean=1, doStoreSK=1)
+    run_driver(str(tmp_path))
+    blk = F.read_out_dat(str(tmp_path / "Out_x.dat"))[0]
+    assert blk.shape == (nbins * Nxb, 2)
+    bnd = [NUMIN + i * (NUMAX - NUMIN) / nbins for i in range(nbins)] + [NUMAX]
+    x = np.concatenate([bnd[b] + np.arange(Nxb) * ((bnd[b + 1] - bnd[b]) / Nxb)
+                        for b in range(nbins)])
+    np.testing.assert_allclose(blk[:, 0], x, rtol=2e-16, atol=0)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    Ko = np.concatenate([oracle_K([sp], [1.0], meanMass, T, P, 1.0, 1.0, cut, bnd[b],
+                                  (bnd[b + 1] - bnd[b]) / Nxb, Nxb, 0.0) for b in range(nbins)])
+    e = rel_err(blk[:, 1], Ko)
+    print("per-bin points: K vs oracle %.3e" % e)
+    assert e <= 1e-11
+    from test_oracle import mean_lines
+    means = np.loadtxt(tmp_path / "Out_x_mean.dat")
+    # the reference multiplies the trapezoid integrals by param dnu once more
+    np.testing.assert_allclose(means, mean_lines(O.means_x(blk[:, 1], blk[:, 0], T), T, DNU),
+                               rtol=1e-12)
+    kb = F.read_out_dat(str(tmp_path / "Out_x_bin.dat"))
+    assert len(kb) == nbins
+    for b in range(nbins):
+        np.testing.assert_array_equal(kb[b][:, 1], np.sort(blk[b * Nxb:(b + 1) * Nxb, 1]))
+
+
+def test_driver_bins_file(tmp_path):
+    """Irregular bins from a bins file on the uniform grid: the per-bin
+    files hold each bin's own number of sorted points (the useIndividualBins
+    branch, heliosk.cu:2292-2345)."""
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 1500, 63, 995.0, 1015.0)
+    bnd = [1000.0, 1001.5, 1004.0, 1007.25, 1010.0]
+    (tmp_path / "bins.txt").write_text("".join("%r\n" % v for v in bnd))
+    (tmp_path / "P.txt").write_text("0.4\n2.5\n")
+    F.write_param_dat(str(tmp_path / "param.dat"), name="b", T=800.0, P=1.0, PFile="P.txt",
+                      Species_Name="01_tst", numin=0.0, numax=1.0, dnu=DNU, cut=5.0,
+                      doStoreFullK=1, nbins=7, binsFile="bins.txt", doStoreSK=2)
+    out = run_driver(str(tmp_path))
+    assert "Use bins in file bins.txt" in out
+    info = open(tmp_path / "Info_b.dat").read()
+    assert "nbins = 4" in info and "use Individual bins: bins.txt" in info
+    blocks = F.read_out_dat(str(tmp_path / "Out_b.dat"))
+    assert len(blocks) == 2 and blocks[0].shape == (10000, 4)
+    for b in range(4):
+        rows = np.array([[float(v) for v in r] for r in _read_rows(tmp_path / ("Out_b_bin%04d.dat" % b))])
+        x = blocks[0][:, 0]
+        sel = (x >= bnd[b]) & (x < bnd[b + 1])
+        N = int(sel.sum())
+        assert rows.shape == (2 * N, 5)
+        for iP in range(2):
+            Kb = np.sort(blocks[iP][sel, 1])
+            r = rows[iP * N:(iP + 1) * N]
+            np.testing.assert_array_equal(r[:, 1], Kb)
+            np.testing.assert_allclose(r[:, 0], [float("%g" % (j / (N - 1.0))) for j in range(N)])
+            np.testing.assert_array_equal(r[:, 4], np.arange(N))

@@ -490,3 +490,24 @@ def test_plinth_removal(profile):
     cover = np.nonzero(np.abs(xj - nu[0]) < st["hw"][0])[0]
     edge = K1[cover[[0, -1]]]
     assert np.all(edge >= -1e-9 * K1.max()) and np.all(edge < 0.05 * K1.max())
+
+
+def test_means_x_trapezoid_closed_forms():
+    T = 1200.0
+    # per-bin grids of 3 bins with different spacings, covering the Planck curve
+    edges = [0.0, 2000.0, 9000.0, 40000.0]
+    Nxb = 200000
+    x = np.concatenate([edges[b] + np.arange(Nxb) * ((edges[b + 1] - edges[b]) / Nxb)
+                        for b in range(3)])
+    K = np.full(x.size, 2.0e-22)
+    s = O.means_x(K, x, T)
+    assert abs(s[0] / s[2] - 2.0e-22) <= 1e-12 * 2.0e-22
+    assert abs(s[3] / s[1] - 2.0e-22) <= 1e-12 * 2.0e-22
+    assert abs(s[2] / (SIGMA * T ** 4 / math.pi) - 1) <= 1e-6      # trapezoid of B, O(dx^2)
+    assert abs(s[3] / (4 * SIGMA * T ** 3 / math.pi) - 1) <= 1e-6
+    # on a uniform grid the trapezoid sums are the plain sums minus half the end terms, times dx
+    xu = 1000.0 + 0.01 * np.arange(5001)
+    Ku = 10 ** np.random.default_rng(1).uniform(-25, -20, xu.size)
+    su, sp = O.means_x(Ku, xu, T), O.means(Ku, 1000.0, 0.01, T)
+    B = O.means(np.ones(1), 1000.0, 1.0, T)[2], O.means(np.ones(1), xu[-1], 1.0, T)[2]
+    assert abs(su[2] - (sp[2] - 0.5 * (B[0] + B[1])) * 0.01) <= 1e-12 * su[2]
