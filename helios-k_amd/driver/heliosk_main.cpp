@@ -28,8 +28,9 @@
 // (OutputEdgesFile).
 //
 // Options the reference has that this driver does not implement are
-// rejected with an error message: CIA and sub-Lorentzian wings.  Profiles
-// 1-4, removePlinth, bins files and "Nnu per bin" grids are supported.
+// rejected with an error message: sub-Lorentzian wings.  Profiles 1-4,
+// removePlinth, bins files, "Nnu per bin" grids and CIA tables (ciaSystem)
+// are supported.
 //
 // Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
@@ -624,6 +625,102 @@ class LineStream {
     std::vector<unsigned char> buf_;
 };
 
+// Collision-induced absorption systems (InitCia, ISO.h:222-276): HITRAN
+// .cia file, number of (T, nu-range) sets, molar mass of the partner [g/mol].
+struct CiaSystem {
+    std::string file;
+    int nsets = 0;
+    double mass1 = 1.0;
+};
+
+bool init_cia(const Param &p, CiaSystem &c) {
+    struct Row { const char *name, *file; int nsets; double mass1; };
+    static const Row rows[] = {
+        {"H2-H2", "H2-H2_2011.cia", 113, 2.0 * 1.00794},
+        {"H2-H2_eq", "H2-H2_eq_2011.cia", 10, 2.0 * 1.00794},
+        {"H2-H2_norm", "H2-H2_norm_2011.cia", 10, 2.0 * 1.00794},
+        {"H2-He", "H2-He_2011.cia", 339, 4.002602},
+        {"H2-He_eq", "H2-He_eq_2011.cia", 10, 4.002602},
+        {"H2-He_norm", "H2-He_norm_2011.cia", 10, 4.002602},
+        {"H2-CH4_eq", "H2-CH4_eq_2011.cia", 10, 16.04246},
+        {"H2-CH4_norm", "H2-CH4_norm_2011.cia", 10, 16.04246},
+        {"H2-H", "H2-H_2011.cia", 4, 1.00794},
+    };
+    for (const Row &r : rows)
+        if (p.ciaSystem == r.name) {
+            c.file = p.path + r.file;
+            c.nsets = r.nsets;
+            c.mass1 = r.mass1;
+            return true;
+        }
+    return fail("Error: cia System not found %s", p.ciaSystem.c_str());
+}
+
+// K from a HITRAN CIA table at (T, P) on the points x (readCiaFile,
+// host.h:670-789): the sets are read in file order; within a set K is linear
+// in nu between the table points; a set with T_set <= T replaces K, a set
+// with T < T_set blends linearly in T with the previous set's K; reading
+// stops at the first set with T <= T_set.  Then K = K rho N_A / mass1 + kmin
+// with rho = P 273.15/T amagat (cm^-3), or kmin where T lies above every
+// set.  The fixed-width header fields (20, 10, 10, 7, 7, 10, 36 characters)
+// are read as the reference reads them.
+bool read_cia(const Param &p, const CiaSystem &cia, const std::vector<double> &x, double P,
+              std::vector<double> &K) {
+    FILE *f = std::fopen(cia.file.c_str(), "r");
+    if (!f) return fail("Error: cia file is not available: %s", cia.file.c_str());
+    const int64_t Nx = (int64_t)x.size();
+    K.assign(Nx, 0.0);
+    const double T = p.T;
+    double Tc0, Tc1 = 0.0;
+    char skip[160], c1[12], c2[12], c3[9], c4[9], c5[12];
+    for (int it = 0; it < cia.nsets; ++it) {
+        Tc0 = Tc1;
+        if (!std::fgets(skip, 21, f) || !std::fgets(c1, 11, f) || !std::fgets(c2, 11, f) ||
+            !std::fgets(c3, 8, f) || !std::fgets(c4, 8, f) || !std::fgets(c5, 11, f) ||
+            !std::fgets(skip, 37, f)) {
+            std::fclose(f);
+            return fail("Error: cia file too short: %s", cia.file.c_str());
+        }
+        const int ncia = std::atoi(c3);
+        Tc1 = std::strtod(c4, nullptr);
+        const double numax = std::strtod(c2, nullptr);
+        double nu0, a0, nu1, a1;
+        if (std::fscanf(f, "%lf %lf %lf %lf", &nu0, &a0, &nu1, &a1) != 4) {
+            std::fclose(f);
+            return fail("Error: cia file too short: %s", cia.file.c_str());
+        }
+        int nc = 2;
+        for (int64_t i = 0; i < Nx; ++i) {
+            while (x[i] >= nu1) {
+                nu0 = nu1;
+                a0 = a1;
+                if (std::fscanf(f, "%lf %lf", &nu1, &a1) != 2) break;
+                ++nc;
+                if (nc == ncia) break;
+            }
+            if (nu0 <= x[i] && x[i] < nu1) {
+                const double k1 = a0 + (a1 - a0) * (x[i] - nu0) / (nu1 - nu0);
+                if (Tc1 <= T) K[i] = k1;
+                if (T < Tc1 && Tc0 > 0.0) K[i] = K[i] + (k1 - K[i]) * (T - Tc0) / (Tc1 - Tc0);
+            }
+            if (nu1 == numax) break;
+            if (i == Nx - 1) {
+                while (nc < ncia) {
+                    if (std::fscanf(f, "%lf %lf", &nu1, &a1) != 2) break;
+                    ++nc;
+                }
+            }
+        }
+        if (!std::fgets(skip, 3, f)) { /* end of file after the last set */ }
+        if (T <= Tc1) break;
+    }
+    std::fclose(f);
+    const double rho = P * 273.15 / T * 2.6867774e19;     // define.h:43, molecules cm^-3
+    for (int64_t i = 0; i < Nx; ++i)
+        K[i] = Tc1 < T ? p.kmin : K[i] * rho * (kNA / cia.mass1) + p.kmin;
+    return true;
+}
+
 struct Timing {
     double read = 0.0, lines = 0.0, voigt = 0.0;
 };
@@ -967,7 +1064,6 @@ bool run(int argc, char **argv) {
     if (p.removePlinth == 1 && p.profile == 4)
         return fail("Error, remove plinth is not supported for profile 4");   // heliosk.cu:410-413
     if (p.subLorentzian != "-") return fail("Error: sub-Lorentzian files are not supported");
-    if (p.ciaSystem != "-") return fail("Error: CIA systems are not supported");
     if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
     if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
         return fail("Error: maxlines must lie in 1 .. 2147483647");
@@ -1110,9 +1206,13 @@ bool run(int argc, char **argv) {
     write_info(stdout, p, Nx, prop.name);
 
     // mean molar mass over species and isotopologues (heliosk.cu:659-686)
-    std::vector<Molecule> mols(spNames.size());
+    // a CIA system replaces the line lists (heliosk.cu:659-705, 800-925)
+    const bool useCia = p.ciaSystem != "-";
+    CiaSystem cia;
+    if (useCia && !init_cia(p, cia)) { std::fclose(info); return false; }
+    std::vector<Molecule> mols(useCia ? 0 : spNames.size());
     double meanMass = 0.0;
-    for (size_t k = 0; k < spNames.size(); ++k) {
+    for (size_t k = 0; k < mols.size(); ++k) {
         if (!init_species(mols[k], p, spNames[k])) { std::fclose(info); return false; }
         if (database_form(mols[k].dataBase) < 0) {
             std::printf("Error: Database %d is not supported\n", mols[k].dataBase);
@@ -1124,7 +1224,7 @@ bool run(int argc, char **argv) {
     std::printf("mean mass %g\n", meanMass);
     double unitScale = 1.0;
     if (p.units == 1) {
-        unitScale = 1.0 / kNA * meanMass;
+        unitScale = 1.0 / kNA * (useCia ? cia.mass1 : meanMass);
         p.kmin /= unitScale;
         std::printf("kmin %g\n", p.kmin);
     }
@@ -1141,6 +1241,11 @@ bool run(int argc, char **argv) {
     auto t_all = std::chrono::steady_clock::now();
     if (p.doStoreFullK >= 0) {
         if (!init_K(p, whole, Plist.size(), K_d, s)) { std::fclose(info); return false; }
+        for (int iP = 0; useCia && iP < p.nP; ++iP) {
+            if (!read_cia(p, cia, x_h, Plist[iP], K_h)) { std::fclose(info); return false; }
+            HIPCHK(hipMemcpy(K_d + (size_t)iP * Nx, K_h.data(), Nx * sizeof(double),
+                             hipMemcpyHostToDevice));
+        }
         for (size_t k = 0; k < mols.size(); ++k) {
             std::vector<double> Q;
             if (!read_partition(mols[k], p.T, Q)) { std::fclose(info); return false; }

@@ -112,6 +112,20 @@ def species_param_name(base, database, nfiles, lo, hi):
     return base + "."
 
 
+def write_cia(path, sets, formula="H2-He"):
+    """HITRAN collision-induced absorption file: per set a 100-character
+    header (formula 20, numin 10, numax 10, point count 7, T 7, max 10 and
+    36 further characters) and "nu cia" rows (the layout readCiaFile,
+    host.h:670-789, reads).  sets: [(T, nu array, cia array)]."""
+    with open(path, "w") as f:
+        for T, nu, cia in sets:
+            head = "%-20s%10.4f%10.4f%7d%7.1f%10.3E" % (formula, nu[0], nu[-1], len(nu), T,
+                                                       max(cia))
+            f.write(head.ljust(100)[:100] + "\n")
+            for a, b in zip(nu, cia):
+                f.write(" %10.4f %.12E\n" % (a, b))
+
+
 def write_param_dat(path, **overrides):
     """param.dat with the reference's keys; overrides use the key text with
     spaces replaced by underscores (e.g. Species_Name="01_hit16")."""

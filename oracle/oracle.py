@@ -255,3 +255,48 @@ def means_x(K, x, T):
     out = np.empty(4)
     lib().hko_means_x(_p(K), _p(x), int(K.size), float(T), _p(out))
     return out
+
+
+def cia_K(path, nsets, x, T, P, mass1, kmin=0.0):
+    """K of a HITRAN CIA table on the points x at (T, P): readCiaFile
+    (host.h:670-789) restated -- sets in file order, linear in nu within a
+    set, replaced by a set with T_set <= T or blended linearly in T from the
+    previous set when T < T_set, stopping at the first set with T <= T_set;
+    then K rho N_A / mass1 + kmin, rho = P 273.15 / T amagat, or kmin above
+    every set.  Pure Python: for small tables."""
+    toks = []
+    with open(path) as f:
+        lines = f.read().split("\n")
+    K = [0.0] * len(x)
+    li = 0
+    Tc1 = 0.0
+    for _ in range(nsets):
+        Tc0 = Tc1
+        h = lines[li]; li += 1
+        ncia = int(h[40:47]); Tc1 = float(h[47:54]); numax = float(h[30:40])
+        pts = []
+        for _k in range(ncia):
+            a, b = lines[li].split(); li += 1
+            pts.append((float(a), float(b)))
+        nu0, a0 = pts[0]; nu1, a1 = pts[1]
+        nc = 2
+        for i, xi in enumerate(x):
+            while xi >= nu1:
+                nu0, a0 = nu1, a1
+                nu1, a1 = pts[nc]
+                nc += 1
+                if nc == ncia:
+                    break
+            if nu0 <= xi < nu1:
+                k1 = a0 + (a1 - a0) * (xi - nu0) / (nu1 - nu0)
+                if Tc1 <= T:
+                    K[i] = k1
+                if T < Tc1 and Tc0 > 0.0:
+                    K[i] = K[i] + (k1 - K[i]) * (T - Tc0) / (Tc1 - Tc0)
+            if nu1 == numax:
+                break
+        if T <= Tc1:
+            break
+    rho = P * 273.15 / T * 2.6867774e19
+    NA = 6.0221412927e23
+    return np.array([kmin if Tc1 < T else k * rho * (NA / mass1) + kmin for k in K])

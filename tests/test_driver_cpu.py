@@ -67,7 +67,6 @@ def _setup(tmp_path, **kw):
     (dict(profile=5), "Error: profile 5 is not supported"),
     (dict(profile=4, removePlinth=1), "Error, remove plinth is not supported for profile 4"),
     (dict(numin=0.0, numax=10.0, dnu=0.01, nbins=7), "Error: range cannot be divided evenly in bins."),
-    (dict(ciaSystem="H2-H2"), "Error: CIA systems are not supported"),
 ])
 def test_driver_rejects_like_the_reference(tmp_path, kw, msg):
     _setup(tmp_path, **kw)

@@ -486,3 +486,32 @@ def test_driver_bins_file(tmp_path):
             np.testing.assert_array_equal(r[:, 1], Kb)
             np.testing.assert_allclose(r[:, 0], [float("%g" % (j / (N - 1.0))) for j in range(N)])
             np.testing.assert_array_equal(r[:, 4], np.arange(N))
+
+
+@pytest.mark.parametrize("T, units", [(1500.0, 0), (700.0, 1), (3200.0, 0)])
+def test_driver_cia(tmp_path, T, units):
+    """ciaSystem = H2-H: K from the HITRAN CIA table (readCiaFile,
+    host.h:670-789) instead of line lists, two pressures, against the
+    oracle's restatement; T above every set gives kmin."""
+    rng = np.random.default_rng(int(T))
+    sets = []
+    for Ts in (500.0, 1000.0, 2000.0, 3000.0):
+        nu = np.round(np.concatenate([[995.0], np.sort(rng.uniform(995.0, 1012.0, 300)), [1012.0]]), 4)
+        sets.append((Ts, nu, 10 ** rng.uniform(-47, -44, nu.size)))
+    F.write_cia(str(tmp_path / "H2-H_2011.cia"), sets, formula="H2-H")
+    (tmp_path / "P.txt").write_text("0.5\n3.0\n")
+    F.write_param_dat(str(tmp_path / "param.dat"), name="c", T=T, P=1.0, PFile="P.txt",
+                      ciaSystem="H2-H", Species_Name="none", numin=NUMIN, numax=NUMAX, dnu=DNU,
+                      cut=5.0, doStoreFullK=1, nbins=NBINS, kmin=1e-40, Units=units)
+    run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    x = NUMIN + DNU * np.arange(nx)
+    blocks = F.read_out_dat(str(tmp_path / "Out_c.dat"))
+    unit = 1.00794 / NA if units else 1.0
+    kmin = 1e-40 / unit
+    for iP, P in enumerate((0.5, 3.0)):
+        Ko = O.cia_K(str(tmp_path / "H2-H_2011.cia"), 4, blocks[iP][:, 0], T, P, 1.00794, kmin)
+        np.testing.assert_allclose(blocks[iP][:, 0], x, rtol=1e-15)
+        np.testing.assert_allclose(blocks[iP][:, 1], Ko * unit, rtol=1e-14, atol=0)
+        if T > 3000.0:
+            assert np.all(blocks[iP][:, 1] == kmin * unit)

@@ -511,3 +511,31 @@ def test_means_x_trapezoid_closed_forms():
     su, sp = O.means_x(Ku, xu, T), O.means(Ku, 1000.0, 0.01, T)
     B = O.means(np.ones(1), 1000.0, 1.0, T)[2], O.means(np.ones(1), xu[-1], 1.0, T)[2]
     assert abs(su[2] - (sp[2] - 0.5 * (B[0] + B[1])) * 0.01) <= 1e-12 * su[2]
+
+
+# ---------------------------------------------------------------- CIA tables
+def test_cia_bilinear_closed_form(tmp_path):
+    """A table sampled from c(nu, T) = a + b nu + c T: linear interpolation in
+    nu within a set and in T between sets reproduces it exactly.  As in the
+    reference, the sweep over the points stops once a set's last table point
+    is loaded, so only the first point at or above the next-to-last table
+    point is filled from the last interval."""
+    from heliosk import files as F
+    a, b, c = 1e-46, 3e-49, 2e-48
+    nu = np.concatenate([[20.0], np.sort(np.random.default_rng(7).uniform(20, 980, 40)), [980.0]])
+    nu = np.round(nu, 4)
+    sets = [(T, nu, a + b * nu + c * T) for T in (200.0, 300.0, 500.0)]
+    p = tmp_path / "t.cia"
+    F.write_cia(str(p), sets)
+    x = np.linspace(0.0, 1000.0, 2001)
+    rho = lambda T, P: P * 273.15 / T * 2.6867774e19
+    NA, m1 = 6.0221412927e23, 4.002602
+    xstar = x[np.searchsorted(x, nu[-2])]
+    inside = (x >= 20.0) & (x <= xstar)
+    for T, P in ((250.0, 1.0), (300.0, 2.0), (420.0, 0.5)):
+        K = O.cia_K(str(p), 3, x, T, P, m1, kmin=1e-30)
+        exact = np.where(inside, (a + b * x + c * T) * rho(T, P) * (NA / m1), 0.0) + 1e-30
+        np.testing.assert_allclose(K, exact, rtol=1e-11)
+    # below the first set: nothing is set; above the last: kmin
+    assert np.all(O.cia_K(str(p), 3, x, 150.0, 1.0, m1, kmin=1e-30) == 1e-30)
+    assert np.all(O.cia_K(str(p), 3, x, 600.0, 1.0, m1, kmin=1e-30) == 1e-30)
