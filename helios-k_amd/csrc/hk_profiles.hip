@@ -16,6 +16,11 @@
 // alpha_D, so a truncated line falls to zero at the cut.  For the Voigt
 // profile the plain Voigt sum runs first (hk_voigt_accumulate, either form)
 // and a second pass subtracts the plinth of every line over its window.
+// Sub-Lorentzian wings (sLChi, voigt.h:47-72, applied at voigt.h:1345-1350 and
+// in every profile branch): inside the given x range each pair's profile
+// value is multiplied by chi(|x_j - nu|) x_j / nu (not the plinth); with the
+// factor on, every profile -- Voigt included, through the FP64 Faddeeva
+// function -- runs in this file's FP64 gather.
 //
 // Kernels: one workgroup of 256 threads per tile of 1024 points (4 per
 // thread, coalesced); the tile's line range comes from the shared
@@ -50,6 +55,7 @@ __device__ __forceinline__ double binned_gauss(double x, double h) {
 // Profile value of one pair; S1 is the per-line prefactor formed at staging.
 template <int PROFILE>
 __device__ __forceinline__ double profile_value(double S1, double x, double y, double vb) {
+    if (PROFILE == 1) return S1 * rew64(x, y);
     if (PROFILE == 2) return S1 / (x * x + y * y);
     if (PROFILE == 3) return S1 * exp(-x * x);
     return S1 / vb * binned_gauss(x, 0.5 * vb);
@@ -57,8 +63,20 @@ __device__ __forceinline__ double profile_value(double S1, double x, double y, d
 
 __device__ __forceinline__ double prefactor(int profile, double S, double y) {
     if (profile == 2) return S * y / kPi;
-    if (profile == 3) return S * kISqrtPi;
+    if (profile == 3 || profile == 1) return S * kISqrtPi;
     return S;
+}
+
+struct Chi {
+    double s1, s2, s3, B1, B2, B3, lo, hi;
+};
+
+// Perrin & Hartmann (1989) chi of the distance d = |x - nu| [cm^-1].
+__device__ __forceinline__ double chi_of(const Chi &c, double d) {
+    if (d < c.s1) return 1.0;
+    if (d < c.s2) return exp(-c.B1 * (d - c.s1));
+    if (d < c.s3) return exp(-c.B1 * (c.s2 - c.s1) - c.B2 * (d - c.s2));
+    return exp(-c.B1 * (c.s2 - c.s1) - c.B2 * (c.s3 - c.s2) - c.B3 * (d - c.s3));
 }
 
 // Plinth of each line: its profile at the cut (0 for empty lines).
@@ -87,11 +105,12 @@ struct PArgs {
     double kmin;
     int accumulate;
     double *K;
+    Chi chi;
 };
 
-// PROFILE 2..4: profile sums (minus plinths when PLINTH); PROFILE 0: the
+// PROFILE 1..4: profile sums (minus plinths when PLINTH); PROFILE 0: the
 // plinth subtraction alone (after a Voigt sum), always accumulating.
-template <int PROFILE, bool PLINTH>
+template <int PROFILE, bool PLINTH, bool CHI>
 __global__ __launch_bounds__(kPT) void profile_tiles_kernel(PArgs a) {
     __shared__ PRec rec[kPT];
     const int tid = threadIdx.x;
@@ -130,8 +149,11 @@ __global__ __launch_bounds__(kPT) void profile_tiles_kernel(PArgs a) {
                 if (jj >= L.lo && jj < L.hi) {
                     double v = 0.0;
                     if (PROFILE) {
-                        const double x = (fma((double)(j0 + jj), a.dnu, a.numin) - L.nu) * L.ia;
-                        v = profile_value<PROFILE>(L.S1, x, L.y, L.vb);
+                        const double xj = fma((double)(j0 + jj), a.dnu, a.numin);
+                        const double d = xj - L.nu;
+                        v = profile_value<PROFILE>(L.S1, d * L.ia, L.y, L.vb);
+                        if (CHI && xj > a.chi.lo && xj < a.chi.hi)
+                            v *= chi_of(a.chi, fabs(d)) * xj / L.nu;
                     }
                     acc[k] += v - L.pl;
                 }
@@ -146,10 +168,16 @@ __global__ __launch_bounds__(kPT) void profile_tiles_kernel(PArgs a) {
     }
 }
 
-template <int PROFILE, bool PLINTH>
+template <int PROFILE, bool PLINTH, bool CHI = false>
 void launch_profile(int64_t ntiles, hipStream_t s, const PArgs &a) {
-    hipLaunchKernelGGL((profile_tiles_kernel<PROFILE, PLINTH>), dim3((unsigned)ntiles), dim3(kPT),
-                       0, s, a);
+    hipLaunchKernelGGL((profile_tiles_kernel<PROFILE, PLINTH, CHI>), dim3((unsigned)ntiles),
+                       dim3(kPT), 0, s, a);
+}
+
+template <int PROFILE>
+void launch_chi(bool plinth, int64_t ntiles, hipStream_t s, const PArgs &a) {
+    if (plinth) launch_profile<PROFILE, true, true>(ntiles, s, a);
+    else launch_profile<PROFILE, false, true>(ntiles, s, a);
 }
 
 size_t tiles_bytes(const hk_grid *g) {
@@ -173,8 +201,8 @@ extern "C" size_t hk_profile_workspace_bytes(const hk_grid *grid, int64_t nlines
 
 extern "C" int hk_profile_accumulate(const hk_line_state *st, const hk_grid *grid, double kmin,
                                      int accumulate, int profile, int remove_plinth,
-                                     int voigt_mode, double *K, void *ws, size_t ws_bytes,
-                                     hk_stream_t stream) {
+                                     int voigt_mode, const hk_chi *chi, double *K, void *ws,
+                                     size_t ws_bytes, hk_stream_t stream) {
     HK_CHECK_ARG(st && grid && K && ws, "hk_profile_accumulate: null argument");
     HK_CHECK_ARG(profile >= HK_PROFILE_VOIGT && profile <= HK_PROFILE_BINNED_GAUSS,
                  "hk_profile_accumulate: profile %d (1 Voigt, 2 Lorentz, 3 Doppler, 4 binned "
@@ -196,7 +224,11 @@ extern "C" int hk_profile_accumulate(const hk_line_state *st, const hk_grid *gri
     char *w = (char *)align_up((size_t)ws, 256);
     const size_t rb = region_bytes(grid);
     double *pl = (double *)(w + rb);
-    if (profile == HK_PROFILE_VOIGT) {
+    if (chi) {
+        HK_CHECK_ARG(chi->sigma[0] <= chi->sigma[1] && chi->sigma[1] <= chi->sigma[2],
+                     "hk_profile_accumulate: chi breaks must be ascending");
+    }
+    if (profile == HK_PROFILE_VOIGT && !chi) {
         hk_tune t = {0, 0};
         int rc = hk_voigt_accumulate(st, grid, kmin, accumulate, voigt_mode, &t, K, w, rb, stream);
         if (rc != HK_OK || !remove_plinth) return rc;
@@ -217,7 +249,20 @@ extern "C" int hk_profile_accumulate(const hk_line_state *st, const hk_grid *gri
     a.nu = st->nu; a.S = st->S; a.ia = st->ialphaD; a.y = st->y; a.hw = st->hw; a.pl = pl;
     a.nl = n; a.tiles = tiles; a.numin = grid->numin; a.dnu = grid->dnu; a.nx = grid->nx;
     a.kmin = kmin; a.accumulate = accumulate; a.K = K;
+    a.chi = Chi{0, 0, 0, 0, 0, 0, 0, 0};
     const bool P = remove_plinth != 0;
+    if (chi) {
+        a.chi = Chi{chi->sigma[0], chi->sigma[1], chi->sigma[2], chi->B[0], chi->B[1], chi->B[2],
+                    chi->range[0], chi->range[1]};
+        switch (profile) {
+        case HK_PROFILE_VOIGT: launch_chi<1>(P, ntiles, s, a); break;
+        case HK_PROFILE_LORENTZ: launch_chi<2>(P, ntiles, s, a); break;
+        case HK_PROFILE_DOPPLER: launch_chi<3>(P, ntiles, s, a); break;
+        default: launch_profile<4, false, true>(ntiles, s, a); break;
+        }
+        HK_LAUNCH_CHECK();
+        return HK_OK;
+    }
     switch (profile) {
     case HK_PROFILE_VOIGT: launch_profile<0, true>(ntiles, s, a); break;
     case HK_PROFILE_LORENTZ:

@@ -28,9 +28,9 @@
 // (OutputEdgesFile).
 //
 // Options the reference has that this driver does not implement are
-// rejected with an error message: sub-Lorentzian wings.  Profiles 1-4,
-// removePlinth, bins files, "Nnu per bin" grids and CIA tables (ciaSystem)
-// are supported.
+// rejected with an error message.  Profiles 1-4, removePlinth, bins files,
+// "Nnu per bin" grids, CIA tables (ciaSystem) and sub-Lorentzian wings
+// (subLorentzianfile) are supported.
 //
 // Extensions (not in the reference): the optional param.dat key
 // "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
@@ -763,7 +763,7 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
                  double meanMass, const std::vector<hk_grid> &pieces,
                  const std::vector<int64_t> &piece_off, int64_t Nx,
                  const std::vector<double> &Plist, double *K_d, hipStream_t s, long long maxlines,
-                 Timing &tm) {
+                 const hk_chi *chi, Timing &tm) {
     // file range (heliosk.cu:939-957)
     int fi0 = m.nFiles, fi1 = 0;
     const double cut = p.cut == 0.0 ? 1.0e30 : p.cut;
@@ -885,7 +885,7 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
             if (!event(e)) { ok = false; break; }
             for (size_t q = 0; q < pieces.size(); ++q)
                 HKCHK(hk_profile_accumulate(&st, &pieces[q], p.kmin, 1, p.profile, plinth, mode,
-                                            K_d + iP * Nx + piece_off[q], dws.p, wsb, s));
+                                            chi, K_d + iP * Nx + piece_off[q], dws.p, wsb, s));
             mk.end.push_back(ev.size());
             if (!event(e)) { ok = false; break; }
         }
@@ -1056,6 +1056,34 @@ bool write_bins_file_K(FILE *kb, const std::string &kname, const Param &p, const
     return true;
 }
 
+// Sub-Lorentzian file (host.h:489-519): sigma1..3, a header, B1..3 rows of
+// alpha beta epsilon, "range in cm^-1" and the range; values are read as
+// float, as there, and B_i = alpha_i + beta_i exp(-epsilon_i T) rounded to
+// float (subLorentzianB, voigt.h:12-35).
+bool read_sublorentz(const Param &p, hk_chi &c) {
+    std::printf("Use sub-Lorentzian file %s\n", p.subLorentzian.c_str());
+    FILE *f = std::fopen(p.subLorentzian.c_str(), "r");
+    if (!f) return fail("Error: sub-Lorentzian file not found: %s", p.subLorentzian.c_str());
+    char b[160];
+    float v[14];
+    int ok = 1;
+    for (int i = 0; i < 3; ++i) ok &= std::fscanf(f, "%159s %f", b, &v[i]) == 2;
+    ok &= std::fscanf(f, "%159s %159s %159s %159s", b, b, b, b) == 4;
+    for (int i = 0; i < 3; ++i)
+        ok &= std::fscanf(f, "%159s %f %f %f", b, &v[3 + 3 * i], &v[4 + 3 * i], &v[5 + 3 * i]) == 4;
+    ok &= std::fscanf(f, "%159s %159s %159s", b, b, b) == 3;
+    ok &= std::fscanf(f, "%f %f", &v[12], &v[13]) == 2;
+    std::fclose(f);
+    if (!ok) return fail("Error in sub-Lorentzian file read %s", p.subLorentzian.c_str());
+    for (int i = 0; i < 3; ++i) {
+        c.sigma[i] = v[i];
+        c.B[i] = (float)(v[3 + 3 * i] + v[4 + 3 * i] * std::exp(-v[5 + 3 * i] * p.T));
+    }
+    c.range[0] = v[12];
+    c.range[1] = v[13];
+    return true;
+}
+
 bool run(int argc, char **argv) {
     Param p;
     if (!read_parameters(p, "param.dat", argc, argv)) return false;
@@ -1063,7 +1091,10 @@ bool run(int argc, char **argv) {
     if (p.profile < 1 || p.profile > 4) return fail("Error: profile %d is not supported", p.profile);
     if (p.removePlinth == 1 && p.profile == 4)
         return fail("Error, remove plinth is not supported for profile 4");   // heliosk.cu:410-413
-    if (p.subLorentzian != "-") return fail("Error: sub-Lorentzian files are not supported");
+    // sub-Lorentzian wings (host.h:489-519; B at T as subLorentzianB, voigt.h:12-35)
+    const bool useSubL = p.subLorentzian != "-";
+    hk_chi chi;
+    if (useSubL && !read_sublorentz(p, chi)) return false;
     if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
     if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
         return fail("Error: maxlines must lie in 1 .. 2147483647");
@@ -1105,6 +1136,9 @@ bool run(int argc, char **argv) {
     // 448-476, setX_kernel voigt.h:751-767)
     const bool useBins = p.bins != "-";
     const bool useX = p.Nxb != 0;
+    // per-bin points: "Nnu per bin", or the sub-Lorentzian option, which the
+    // reference runs on its per-bin point kernels (heliosk.cu:477-481)
+    const bool perBinX = useX || useSubL;
     std::vector<double> bnd;
     if (useBins) {
         std::printf("Use bins in file %s\n", p.bins.c_str());
@@ -1154,7 +1188,7 @@ bool run(int argc, char **argv) {
     std::vector<hk_grid> pieces;
     std::vector<int64_t> piece_off;
     std::vector<double> x_h(Nx);
-    if (!useX) {
+    if (!perBinX) {
         pieces.push_back(hk_grid{p.numin, p.dnu, Nx});
         piece_off.push_back(0);
         for (int64_t j = 0; j < Nx; ++j) x_h[j] = std::fma((double)j, p.dnu, p.numin);
@@ -1230,6 +1264,7 @@ bool run(int argc, char **argv) {
     }
 
     const hk_grid whole = {0.0, 1.0, Nx};     // index space of all Nx points
+    const hk_grid uni = {p.numin, p.dnu, Nx};   // the equal bins' uniform grid
     hipStream_t s;
     HIPCHK(hipStreamCreate(&s));
     DevBuf Kd;
@@ -1252,7 +1287,7 @@ bool run(int argc, char **argv) {
             const double Sscale = p.nSpecies > 1 ? spAb[k] : 1.0;
             std::printf("mean mass %g, Sscale %g\n", meanMass, Sscale);
             if (!add_species(p, mols[k], Q, Sscale, meanMass, pieces, piece_off, Nx, Plist, K_d,
-                             s, p.maxlines, tm)) {
+                             s, p.maxlines, useSubL ? &chi : nullptr, tm)) {
                 std::fclose(info);
                 return false;
             }
@@ -1336,18 +1371,17 @@ bool run(int argc, char **argv) {
         const double integral1 = sigma * T * T * T * T / M_PI;
         const double integral2 = M_PI / (4.0 * sigma * T * T * T);
         DevBuf xd;
-        if (useX) {
+        if (perBinX) {
             if (!xd.alloc(Nx * sizeof(double))) return fail("Error: allocation for the means failed");
             HIPCHK(hipMemcpy(xd.p, x_h.data(), Nx * sizeof(double), hipMemcpyHostToDevice));
         }
         for (int iP = 0; iP < p.nP; ++iP) {
             double m[4];
-            if (useX)   // trapezoid rule on the per-bin points (resample.h:700-716)
+            if (perBinX)   // trapezoid rule on the per-bin points (resample.h:700-716)
                 HKCHK(hk_means_x(K_d + (size_t)iP * Nx, xd.as<double>(), Nx, T, msum.as<double>(),
                                  mws.p, mwb, s));
             else
-                HKCHK(hk_means(K_d + (size_t)iP * Nx, &pieces[0], T, msum.as<double>(), mws.p,
-                               mwb, s));
+                HKCHK(hk_means(K_d + (size_t)iP * Nx, &uni, T, msum.as<double>(), mws.p, mwb, s));
             HIPCHK(hipMemcpyAsync(m, msum.p, sizeof(m), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (p.nP == 1) {
@@ -1428,7 +1462,7 @@ bool run(int argc, char **argv) {
         if (!ws.alloc(wsb) || !coeff.alloc((size_t)p.nbins * nC * sizeof(double)) ||
             !nxmin.alloc((size_t)p.nbins * sizeof(int32_t)) || !Ks.alloc(Nx * sizeof(double)))
             return fail("Error: k-table allocation failed");
-        HKCHK(hk_ktable_prepare(&pieces[0], p.numax, p.nbins, nC, ws.p, wsb));
+        HKCHK(hk_ktable_prepare(&uni, p.numax, p.nbins, nC, ws.p, wsb));
         if (need_recon)
             if (!Ky.alloc(Nx * sizeof(double)) || !scr.alloc(Nx * sizeof(double)))
                 return fail("Error: k-table allocation failed");
@@ -1456,7 +1490,7 @@ bool run(int argc, char **argv) {
         nxmin_h.resize(p.nbins);
         for (int iP = 0; iP < p.nP; ++iP) {
             double *KP = K_d + (size_t)iP * Nx;
-            HKCHK(hk_ktable(KP, 1, &pieces[0], p.numax, p.nbins, nC, p.kmin, unitScale,
+            HKCHK(hk_ktable(KP, 1, &uni, p.numax, p.nbins, nC, p.kmin, unitScale,
                             coeff.as<double>(), nxmin.as<int32_t>(), Ks.as<double>(), ws.p, wsb,
                             s));
             HIPCHK(hipMemcpyAsync(coeff_h.data(), coeff.p, coeff_h.size() * sizeof(double),

@@ -40,6 +40,11 @@ class Cut(ctypes.Structure):
     _fields_ = [("cut_mode", c_int), ("cut", c_d)]
 
 
+class Chi(ctypes.Structure):
+    _fields_ = [("sigma", ctypes.c_double * 3), ("B", ctypes.c_double * 3),
+                ("range", ctypes.c_double * 2)]
+
+
 class Grid(ctypes.Structure):
     _fields_ = [("numin", c_d), ("dnu", c_d), ("nx", c_i64)]
 
@@ -92,7 +97,7 @@ def lib():
         L.hk_profile_workspace_bytes.restype = c_size
         L.hk_profile_workspace_bytes.argtypes = [P(Grid), c_i64]
         L.hk_profile_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, c_int, c_int,
-                                            c_vp, c_vp, c_size, c_vp]
+                                            P(Chi), c_vp, c_vp, c_size, c_vp]
         L.hk_means_x.argtypes = [c_vp, c_vp, c_i64, c_d, c_vp, c_vp, c_size, c_vp]
         L.hk_sort_bins_workspace_bytes.restype = c_size
         L.hk_sort_bins_workspace_bytes.argtypes = [c_i64, c_int]

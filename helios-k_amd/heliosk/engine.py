@@ -204,16 +204,24 @@ def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, acc
 
 
 def profile(state, grid, profile, remove_plinth=False, kmin=0.0, voigt_mode=HK_VOIGT_FP64, K=None,
-            accumulate=False, stream=None):
+            accumulate=False, chi=None, stream=None):
     """K_j = kmin + sum_i profile_i(x_j) (minus each line's plinth) for
-    profile 1 (Voigt), 2 (Lorentz), 3 (Doppler) or 4 (binned Gaussian)."""
+    profile 1 (Voigt), 2 (Lorentz), 3 (Doppler) or 4 (binned Gaussian);
+    chi = (sigma1..3, B1..3, range_lo, range_hi) applies sub-Lorentzian wings."""
+    c = None
+    if chi is not None:
+        c = _lib.Chi()
+        for k in range(3):
+            c.sigma[k] = float(chi[k]); c.B[k] = float(chi[3 + k])
+        c.range[0] = float(chi[6]); c.range[1] = float(chi[7])
     if K is None:
         K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
     nbytes = lib().hk_profile_workspace_bytes(ctypes.byref(grid), state.nlines)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
     check(lib().hk_profile_accumulate(ctypes.byref(state.st), ctypes.byref(grid), float(kmin),
                                       int(bool(accumulate)), int(profile), int(bool(remove_plinth)),
-                                      int(voigt_mode), _ptr(K), _ptr(ws), nbytes, _stream(stream)))
+                                      int(voigt_mode), None if c is None else ctypes.byref(c),
+                                      _ptr(K), _ptr(ws), nbytes, _stream(stream)))
     return K
 
 

@@ -269,11 +269,25 @@ int hk_ktable_plan_host(int Nxb, int nC, double *P);
  * call).  Profiles 2-4 and the plinth are FP64.  Workspace: the Voigt
  * workspace or the tile table, whichever is larger, plus one double per
  * line. */
+/* Sub-Lorentzian line wings (Perrin & Hartmann 1989; the subLorentzianfile
+ * option, host.h:489-519, sLChi voigt.h:47-72): for range[0] < x < range[1]
+ * every pair's profile value is multiplied by chi(|x - nu|) x / nu, with
+ * chi = 1 below sigma[0], then exp(-B[0](d - sigma[0])), continued with
+ * slopes B[1] above sigma[1] and B[2] above sigma[2]; B at the run's T. */
+typedef struct {
+    double sigma[3];
+    double B[3];
+    double range[2];
+} hk_chi;
+
 size_t hk_profile_workspace_bytes(const hk_grid *grid, int64_t nlines);
+/* chi: NULL, or the sub-Lorentzian factor; with it every profile (Voigt
+ * through the FP64 Faddeeva function, voigt_mode then unused) runs in the
+ * FP64 gather. */
 int hk_profile_accumulate(const hk_line_state *st, const hk_grid *grid, double kmin,
                           int accumulate, int profile, int remove_plinth, int voigt_mode,
-                          double *K, void *workspace, size_t workspace_bytes,
-                          hk_stream_t stream);
+                          const hk_chi *chi, double *K, void *workspace,
+                          size_t workspace_bytes, hk_stream_t stream);
 
 /* Planck and Rosseland mean sums of one spectrum on a uniform grid
  * (Mean_kernel + IntegrateMean_kernel, resample.h:635-771, as main calls

@@ -40,6 +40,11 @@
  *      telescope), the Doppler and Lorentz sums approach S (and the
  *      Lorentz tail's S y / (pi x)), the plinth-removed profile vanishes at
  *      the cut and equals the plain one minus a constant per line.
+ *  hko_chi (and the chi argument of hko_profile_points)
+ *      Sub-Lorentzian wing factor (sLChi, voigt.h:47-72, applied as
+ *      chi x / nu at voigt.h:1345-1350).  Pinned by its properties:
+ *      chi = 1 below sigma1, continuous at sigma1..3, log-slope -B_i in
+ *      segment i, and the x / nu factor between points at equal |x - nu|.
  *  hko_voigt_grad
  *      Analytic gradient of sum_j gK_j K_j; pinned by central finite
  *      differences of hko_voigt_points.
@@ -336,11 +341,25 @@ double hko_profile_value(int profile, double S, double x, double y, double vb)
     }
 }
 
+/* Perrin & Hartmann (1989) sub-Lorentzian factor chi(|x - nu|) (sLChi,
+ * voigt.h:47-72): 1 below sigma1, then exp(-B1 (d - sigma1)), continued with
+ * slopes B2 above sigma2 and B3 above sigma3.  chi = {sigma1..3, B1..3,
+ * range_lo, range_hi}. */
+double hko_chi(const double *chi, double d)
+{
+    const double s1 = chi[0], s2 = chi[1], s3 = chi[2], B1 = chi[3], B2 = chi[4], B3 = chi[5];
+    if (d < s1) return 1.0;
+    if (d < s2) return exp(-B1 * (d - s1));
+    if (d < s3) return exp(-B1 * (s2 - s1) - B2 * (d - s2));
+    return exp(-B1 * (s2 - s1) - B2 * (s3 - s2) - B3 * (d - s3));
+}
+
 void hko_profile_points(int64_t nl, const double *nu, const double *S,
                         const double *ialphaD, const double *y,
                         const double *hw, double R,
                         double numin, double dnu, int64_t Nx, double kmin,
-                        int profile, int remove_plinth, double *K, int nthreads)
+                        int profile, int remove_plinth, const double *chi, double *K,
+                        int nthreads)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -359,6 +378,9 @@ void hko_profile_points(int64_t nl, const double *nu, const double *S,
             if (!(d < hw[i] && d > -hw[i])) continue;
             double vb = dnu * ialphaD[i];
             double t = hko_profile_value(profile, S[i], d * ialphaD[i], y[i], vb);
+            /* sub-Lorentzian wings: chi(|x - nu|) x / nu inside the range
+               (voigt.h:1345-1350) */
+            if (chi && xj > chi[6] && xj < chi[7]) t *= hko_chi(chi, fabs(d)) * xj / nu[i];
             if (remove_plinth)
                 t -= hko_profile_value(profile, S[i], hw[i] * ialphaD[i], y[i], vb);
             double s2 = sum + t;

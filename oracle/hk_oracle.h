@@ -61,11 +61,13 @@ void hko_voigt_points(int64_t nl, const double *nu, const double *S,
 
 /* ---- profiles 1-4 with optional plinth removal (all grid points) ---- */
 double hko_profile_value(int profile, double S, double x, double y, double vb);
+double hko_chi(const double *chi, double d);
 void hko_profile_points(int64_t nl, const double *nu, const double *S,
                         const double *ialphaD, const double *y,
                         const double *hw, double search_radius,
                         double numin, double dnu, int64_t Nx, double kmin,
-                        int profile, int remove_plinth, double *K, int nthreads);
+                        int profile, int remove_plinth, const double *chi, double *K,
+                        int nthreads);
 
 /* ---- gradient of L = sum_j gK_j K_j w.r.t. T, P, S0_i, vy0_i ---- */
 void hko_voigt_grad(int64_t nl,

@@ -55,7 +55,9 @@ def lib():
         L.hko_profile_value.restype = c_d
         L.hko_profile_value.argtypes = [c_int, c_d, c_d, c_d, c_d]
         L.hko_profile_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d, c_int,
-                                                              c_int, _D, c_int])
+                                                              c_int, _D, _D, c_int])
+        L.hko_chi.restype = c_d
+        L.hko_chi.argtypes = [_D, c_d]
         L.hko_means_x.argtypes = [_D, _D, c_i64, c_d, _D]
         L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
         _lib = L
@@ -116,9 +118,11 @@ def line_params(lines, T, P, cut_mode, cut, form=0):
     return out
 
 
-def profile_points(state, numin, dnu, Nx, profile, remove_plinth=False, kmin=0.0, nthreads=0):
+def profile_points(state, numin, dnu, Nx, profile, remove_plinth=False, kmin=0.0, nthreads=0,
+                   chi=None):
     """K on all Nx points for profile 1 (Voigt), 2 (Lorentz), 3 (Doppler) or
-    4 (binned Gaussian), optionally with each line's plinth removed."""
+    4 (binned Gaussian), optionally with each line's plinth removed and the
+    sub-Lorentzian factor chi = (sigma1..3, B1..3, range_lo, range_hi)."""
     cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
     if cols[0].size > 1 and np.any(np.diff(cols[0]) < 0):
         o = np.argsort(cols[0], kind="stable")
@@ -128,9 +132,13 @@ def profile_points(state, numin, dnu, Nx, profile, remove_plinth=False, kmin=0.0
     R = float(hw[np.isfinite(hw)].max()) if nl else 0.0
     K = np.empty(int(Nx))
     lib().hko_profile_points(nl, *[_p(c) for c in cols], R, float(numin), float(dnu), int(Nx),
-                             float(kmin), int(profile), int(bool(remove_plinth)), _p(K),
-                             int(nthreads))
+                             float(kmin), int(profile), int(bool(remove_plinth)),
+                             None if chi is None else _p(_f64(chi)), _p(K), int(nthreads))
     return K
+
+
+def chi_factor(chi, d):
+    return lib().hko_chi(_p(_f64(chi)), float(d))
 
 
 def profile_value(profile, S, x, y, vb):

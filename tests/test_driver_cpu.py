@@ -136,6 +136,17 @@ def test_driver_bins_and_per_bin_points_errors(tmp_path):
     assert "Error; bin boundaries not monotonic growing" in out, out
 
 
+def test_driver_sublorentz_file_errors(tmp_path):
+    _setup(tmp_path, subLorentzianfile="nochi.dat")
+    rc, out = run_driver(str(tmp_path))
+    assert "Use sub-Lorentzian file nochi.dat" in out
+    assert "Error: sub-Lorentzian file not found: nochi.dat" in out, out
+    (tmp_path / "chi.dat").write_text("sigma1 3.0\nsigma2 30.0\n")
+    _setup(tmp_path, subLorentzianfile="chi.dat")
+    rc, out = run_driver(str(tmp_path))
+    assert "Error in sub-Lorentzian file read chi.dat" in out, out
+
+
 def test_driver_undefined_key(tmp_path):
     (tmp_path / "param.dat").write_text("name = a\nT = 300\nnot a key = 3\n")
     rc, out = run_driver(str(tmp_path))

@@ -515,3 +515,40 @@ def test_driver_cia(tmp_path, T, units):
         np.testing.assert_allclose(blocks[iP][:, 1], Ko * unit, rtol=1e-14, atol=0)
         if T > 3000.0:
             assert np.all(blocks[iP][:, 1] == kmin * unit)
+
+
+def test_driver_sublorentzian(tmp_path):
+    """subLorentzianfile: Perrin-Hartmann wings inside the file's range, on
+    the per-bin points the reference uses with this option; against the
+    oracle on each bin's grid with B_i at T rounded to float as there."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 1500, 71, 2180.0, 2320.0)
+    (tmp_path / "chi.dat").write_text(
+        "sigma1 3.0\nsigma2 30.0\nsigma3 120.0\n\nB\talpha \tbeta \tepsilon\n"
+        "B1\t0.0888\t-0.160\t0.00410\nB2\t0.0\t0.0526\t0.00152\nB3\t0.0232\t0.0\t0.0\n\n"
+        "range in cm^-1\n2210 2290\n")
+    T, P, cut, numin, numax, dnu, nbins = 650.0, 3.0, 40.0, 2200.0, 2300.0, 0.01, 4
+    F.write_param_dat(str(tmp_path / "param.dat"), name="s", T=T, P=P, Species_Name="01_tst",
+                      numin=numin, numax=numax, dnu=dnu, cut=cut, doStoreFullK=1, nbins=nbins,
+                      subLorentzianfile="chi.dat")
+    out = run_driver(str(tmp_path))
+    assert "Use sub-Lorentzian file chi.dat" in out
+    blk = F.read_out_dat(str(tmp_path / "Out_s.dat"))[0]
+    f32 = lambda v: float(np.float32(v))
+    ab = [(f32(0.0888), f32(-0.160), f32(0.00410)), (0.0, f32(0.0526), f32(0.00152)),
+          (f32(0.0232), 0.0, 0.0)]
+    chi = [3.0, 30.0, 120.0] + [f32(a + b * np.exp(-e * T)) for a, b, e in ab] + [2210.0, 2290.0]
+    nx = int((numax - numin) / dnu + 0.5)
+    Nxb = nx // nbins
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    L, m = sp["lines"], sp["masses"]
+    d = dict(nu=L["nu"], S=L["S"], EL=L["EL"], A=L["A"], delta=L["delta"], vy=L["gself"],
+             n=L["n"], iso=sp["iso"].astype(np.int32), iso_mass=m / NA,
+             iso_ab=sp["abund"] * m / meanMass, iso_sscale=m / meanMass,
+             iso_Q=np.array([q_at(T, Q) for Q in sp["Qtab"]]))
+    st = O.line_params(d, T, P, 0, cut)
+    bnd = [numin + i * (numax - numin) / nbins for i in range(nbins)] + [numax]
+    Ko = np.concatenate([O.profile_points(st, bnd[b], (bnd[b + 1] - bnd[b]) / Nxb, Nxb, 1, chi=chi)
+                         for b in range(nbins)])
+    e = rel_err(blk[:, 1], Ko)
+    print("driver sub-Lorentzian vs oracle %.3e" % e)
+    assert e <= 1e-11

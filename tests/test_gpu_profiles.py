@@ -74,3 +74,29 @@ def test_profile_accumulate_and_errors(E):
         E.profile(st, grid, 4, remove_plinth=True)
     with pytest.raises(Exception, match="profile 5"):
         E.profile(st, grid, 5)
+
+
+def _chi(T, lo, hi):
+    ab = [(0.0888, -0.160, 0.00410), (0.0, 0.0526, 0.00152), (0.0232, 0.0, 0.0)]
+    return [3.0, 30.0, 120.0] + [a + b * np.exp(-e * T) for a, b, e in ab] + [lo, hi]
+
+
+@pytest.mark.parametrize("profile, plinth", [(1, False), (1, True), (2, False), (3, True),
+                                             (4, False)])
+def test_sublorentzian_profiles_vs_oracle(E, profile, plinth):
+    d = random_lines(2500, seed=30 + profile, numin=2150.0, numax=2350.0, A=1e6,
+                     gamma=(0.05, 0.2))
+    st = E.line_params(species_from(d), 700.0, 5.0, Q=d["iso_Q"], cut_mode=0, cut=60.0)
+    grid = E.make_grid(2180.0, 2320.0, 0.01)
+    chi = _chi(700.0, 2200.0, 2300.0)
+    K = E.profile(st, grid, profile, remove_plinth=plinth, chi=chi).cpu().numpy()
+    Ko = O.profile_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, profile,
+                          remove_plinth=plinth, chi=chi)
+    e = rel_err(K, Ko)
+    print("sub-Lorentzian profile %d plinth %d: %.3e" % (profile, plinth, e))
+    assert e <= 1e-12
+    Kplain = O.profile_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, profile,
+                              remove_plinth=plinth)
+    x = grid.numin + grid.dnu * np.arange(grid.nx)
+    out = (x <= 2200.0) | (x >= 2300.0)
+    assert np.max(np.abs(Ko[~out] - Kplain[~out])) > 0    # the factor acts inside the range

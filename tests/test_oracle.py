@@ -539,3 +539,50 @@ def test_cia_bilinear_closed_form(tmp_path):
     # below the first set: nothing is set; above the last: kmin
     assert np.all(O.cia_K(str(p), 3, x, 150.0, 1.0, m1, kmin=1e-30) == 1e-30)
     assert np.all(O.cia_K(str(p), 3, x, 600.0, 1.0, m1, kmin=1e-30) == 1e-30)
+
+
+# ---------------------------------------------------------------- sub-Lorentzian wings
+CHI_CO2 = [3.0, 30.0, 120.0]    # sigma1..3 of chiPH89__CO2-CO2.dat
+
+
+def _chi_params(T, lo=2100.0, hi=2500.0):
+    """B_i = alpha_i + beta_i exp(-epsilon_i T) with the CO2-CO2 coefficients."""
+    ab = [(0.0888, -0.160, 0.00410), (0.0, 0.0526, 0.00152), (0.0232, 0.0, 0.0)]
+    B = [a + b * math.exp(-e * T) for a, b, e in ab]
+    return CHI_CO2 + B + [lo, hi]
+
+
+def test_chi_properties():
+    chi = _chi_params(296.0)
+    s1, s2, s3, B1, B2, B3 = chi[:6]
+    assert all(O.chi_factor(chi, d) == 1.0 for d in (0.0, 1.0, 2.999))
+    eps = 1e-9
+    for s in (s1, s2, s3):                         # continuous at each break
+        a, b = O.chi_factor(chi, s - eps), O.chi_factor(chi, s + eps)
+        assert abs(a - b) <= 1e-8 * a
+    for lo, hi, B in ((s1, s2, B1), (s2, s3, B2), (s3, 400.0, B3)):   # log-slope -B_i
+        d0, d1 = lo + 0.3 * (hi - lo), lo + 0.7 * (hi - lo)
+        slope = (math.log(O.chi_factor(chi, d1)) - math.log(O.chi_factor(chi, d0))) / (d1 - d0)
+        assert abs(slope + B) <= 1e-12 * max(B, 1e-3)
+
+
+def test_sublorentzian_profile_factor():
+    """One Lorentz line in the range: K_chi = K chi(|x - nu|) x / nu; outside
+    the range, or for |x - nu| < sigma1, K is unchanged."""
+    chi = _chi_params(500.0, lo=2200.0, hi=2300.0)
+    numin, dnu, Nx = 2150.0, 0.05, 4001
+    st = dict(nu=np.array([2250.0]), S=np.array([1e-20 * 30.0]), ialphaD=np.array([30.0]),
+              y=np.array([3.0]), hw=np.array([90.0]))
+    K = O.profile_points(st, numin, dnu, Nx, 2)
+    Kc = O.profile_points(st, numin, dnu, Nx, 2, chi=chi)
+    x = numin + dnu * np.arange(Nx)
+    d = np.abs(x - 2250.0)
+    inr = (x > 2200.0) & (x < 2300.0) & (d < 90.0)
+    np.testing.assert_array_equal(Kc[~inr], K[~inr])
+    near = inr & (d < 3.0)
+    np.testing.assert_allclose(Kc[near], K[near] * x[near] / 2250.0, rtol=1e-15)
+    # two points at the same distance on either side: the ratio is x1/x2
+    j1, j2 = np.searchsorted(x, 2250.0 - 40.0), np.searchsorted(x, 2250.0 + 40.0)
+    r1, r2 = Kc[j1] / K[j1], Kc[j2] / K[j2]
+    assert abs((r1 / r2) / (x[j1] / x[j2]) - 1) <= 1e-13
+    assert r2 < 1.0                               # wings are sub-Lorentzian
