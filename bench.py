@@ -7,7 +7,11 @@ coefficients -- the precision-at-scale configuration.  One step = line
 parameters + FP32-compensated Voigt sum + k-table over the whole grid, with
 the line list already resident in HBM.  value = algorithmic Voigt
 evaluations (line-point pairs inside the cut windows, counted exactly by the
-library) per second of step time.
+library) per second of step time -- the pairs the reference evaluates one
+by one.  The FP32C form sums the lines far from a tile through an 11-node
+Chebyshev far field; "voigt_work" gives the pairs evaluated directly and
+those delivered through the far field, and the roofline counts only the
+operations actually done.
 
 Also reported: the plain FP64 form's evaluations/s on the same input, the
 k-table points/s, FP32C-vs-FP64 max relative error on the full grid, the
@@ -30,6 +34,12 @@ for p in (ROOT, os.path.join(ROOT, "helios-k_amd")):
 # reference's regions (A: r2 >= 1e6 one-term Lorentz wing; B: 100 <= r2 < 1e6
 # rational; C: r2 < 100 Algorithm 916 with ~9 series terms).
 FLOPS_PER_EVAL = {"A": 10.0, "B": 22.0, "C": 400.0}
+# Far field of the FP32C form: one FP64 node evaluation (7-term complex
+# asymptotic series with a refined reciprocal, ~70 FP64 operations, counted
+# twice at the FP32 rate) per node; the 11-term Clenshaw sum per point.
+FAR_NODES = 11
+FLOPS_PER_NODE = 140.0
+FLOPS_PER_POINT_FAR = 25.0
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
 PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
 PEAK_HBM_GBS = 8000.0
@@ -125,8 +135,13 @@ def main():
     counts = E.pair_count(st, grid)
     evals = counts[0]
     region = {"A": counts[1], "B": counts[2], "C": counts[3]}
-    flops = sum(region[r] * FLOPS_PER_EVAL[r] for r in region)
+    work = E.work_count(st, grid, E.HK_VOIGT_FP32C)
+    direct = {"A": work[0], "B": work[1], "C": work[2]}
+    flops = (sum(direct[r] * FLOPS_PER_EVAL[r] for r in direct)
+             + work[4] * FAR_NODES * FLOPS_PER_NODE + grid.nx * FLOPS_PER_POINT_FAR)
     log("bench: %.4e evaluations per step, regions A/B/C = %s" % (evals, region))
+    log("bench: FP32C work: direct pairs A/B/C = %s, far-field pairs %.4e, far (line, tile) "
+        "%.4e" % (direct, work[3], work[4]))
 
     def step(mode, ev=None):
         E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut, state=st)
@@ -163,6 +178,8 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     ktable_ms = e0.elapsed_time(e1) / 3
+
+    extra = other_paths(E, torch, arr, sp, st, grid, K32, kt, w, T, evals)
 
     # plain FP64 form on the same input
     fp64_ms = None
@@ -205,6 +222,9 @@ def main():
                    "bins": w.nbins, "nC": w.nC, "parallelism": "single GPU"},
         "voigt_evals_per_step": evals,
         "voigt_regions_ABC": [region["A"], region["B"], region["C"]],
+        "voigt_work": {"direct_pairs_ABC": [direct["A"], direct["B"], direct["C"]],
+                       "far_field_pairs": work[3], "far_line_tiles": work[4],
+                       "far_node_evaluations": work[4] * FAR_NODES},
         "voigt_kernel_ms": voigt_ms,
         "voigt_evals_per_s_kernel": evals / (voigt_ms * 1e-3),
         "fp64_voigt_kernel_ms": fp64_ms,
@@ -215,16 +235,60 @@ def main():
         "ktable_ms": ktable_ms,
         "ktable_points_per_s": grid.nx / (ktable_ms * 1e-3),
         "ktable_hbm_GBs_algorithmic": 8.0 * grid.nx / (ktable_ms * 1e-3) / 1e9,
+        "other_paths": extra,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": hbm_traffic(VOIGT_KERNELS, args.workload),
                      "kernel": "one Voigt sum = main pass + line-core pass (%s)"
                                % " + ".join(VOIGT_KERNELS),
-                     "model": "ops/eval A=%g B=%g C=%g (FP64 formulas, div/exp/trig x4)"
-                              % (FLOPS_PER_EVAL["A"], FLOPS_PER_EVAL["B"], FLOPS_PER_EVAL["C"])},
+                     "model": "ops/direct pair A=%g B=%g C=%g (FP64 formulas, div/exp/trig x4) "
+                              "+ %g per far-field node + %g per point"
+                              % (FLOPS_PER_EVAL["A"], FLOPS_PER_EVAL["B"], FLOPS_PER_EVAL["C"],
+                                 FLOPS_PER_NODE, FLOPS_PER_POINT_FAR)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def other_paths(E, torch, arr, sp, st, grid, K, kt, w, T, evals):
+    """The paths around the Voigt sum on the same workload, timed with
+    events (3 runs each): device line sort of the raw list, Planck/Rosseland
+    sums, transmission of the sorted bins, and the Lorentz profile gather."""
+    out = {}
+
+    def timed(fn, n=3):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n
+
+    dev = K.device
+    raw = {k: torch.from_numpy(arr[k]).to(dev) for k in ("nu", "S", "EL", "vy", "n")}
+    n = sp.nlines
+    ms = timed(lambda: E.sort_lines(raw["nu"], raw["S"], raw["EL"], raw["vy"], raw["n"],
+                                    iso_mass=arr["iso_mass"]))
+    out["line_sort"] = {"ms": ms, "lines_per_s": n / (ms * 1e-3),
+                        "note": "raw list (unsorted, 7 FP64 fields) -> stable radix sort by nu "
+                                "+ gather; includes output allocation"}
+    ms = timed(lambda: E.means(K, grid, T), n=5)
+    out["means"] = {"ms": ms, "points_per_s": grid.nx / (ms * 1e-3),
+                    "hbm_GBs_algorithmic": 8.0 * grid.nx / (ms * 1e-3) / 1e9}
+    _, _, Ks = kt(K, kmin=0.0, want_sorted=True)
+    nTr, Nxb = 1000, grid.nx // w.nbins
+    ms = timed(lambda: E.transmission(Ks, w.nbins, Nxb, nTr, 0.05))
+    out["transmission"] = {"ms": ms, "nTr": nTr,
+                           "exp_per_s": w.nbins * Nxb * nTr / (ms * 1e-3)}
+    Kl = torch.empty_like(K)
+    ms = timed(lambda: E.profile(st, grid, 2, K=Kl), n=1)
+    out["lorentz_profile"] = {"ms": ms, "pairs_per_s": evals / (ms * 1e-3),
+                              "note": "FP64 gather, same cut windows as the Voigt sum"}
+    log("bench: other paths %s" % json.dumps({k: round(v["ms"], 3) for k, v in out.items()}))
+    return out
 
 
 def bench_batched(args, w):

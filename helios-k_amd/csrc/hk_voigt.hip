@@ -35,6 +35,8 @@
 //                  asymptotic); per-point FP32 partial sums over 32 lines
 //                  folded into a two-sum (hi, lo) pair; strengths scaled by
 //                  2^-e per tile so the largest maps to 2^60.
+//                  Far lines (see tile_f32c) are summed at 11 Chebyshev
+//                  nodes of the tile in FP64 and interpolated to its points.
 #include <math.h>
 
 #include <algorithm>
@@ -58,6 +60,7 @@ struct VArgs {
     int64_t nx;
     double kmin;
     int accumulate;
+    int far;                                // FP32 form: far-field interpolation on
     double *K;
 };
 
@@ -409,8 +412,97 @@ __device__ __forceinline__ void dispatch32(const Rec32 &q, int code, float *part
     }
 }
 
+// Position of this thread's valid record among the workgroup's valid
+// records (line order) and their total; cnt[] is 4 ints of LDS.
+__device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(valid);
+    const int below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) cnt[wave] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int c = cnt[w];
+        off += w < wave ? c : 0;
+        total += c;
+    }
+    return off + below;
+}
+
+// ---------------------------------------------------------------------------
+// Far field of the FP32 form.  A line whose cut window covers the whole tile
+// and whose centre lies at least kFarRatio tile half-widths from the tile's
+// centre contributes a function that is analytic over the tile with its
+// nearest singularity (the line centre) far outside it: its Chebyshev
+// interpolant on kFN nodes is exact to ~4e-9 relative (Bernstein-ellipse
+// bound rho^-10, rho = 4 + sqrt(15)).  Such lines are not evaluated point by
+// point; each adds its FP64 values at the kFN nodes, and the sum over them is
+// interpolated to the tile's points once.  The node values use the
+// asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n z^-2n, c_n = (2n-1)!!/2^n,
+// n <= 6, which requires x^2 + y^2 >= kFarR2 over the tile (truncation
+// < 1e-14 relative).
+// ---------------------------------------------------------------------------
+constexpr int kFN = 11;
+constexpr double kFarRatio = 4.0;
+constexpr double kFarR2 = 400.0;
+constexpr int kFarMinPts = 256;       // smaller (edge) tiles sum every pair
+// Chebyshev-Gauss nodes cos(pi (k + 1/2) / kFN)
+__constant__ double kFarU[kFN] = {
+    0.98982144188093273238, 0.90963199535451837141, 0.75574957435425828377,
+    0.54064081745559758211, 0.28173255684142969771, 0.0, -0.28173255684142969771,
+    -0.54064081745559758211, -0.75574957435425828377, -0.90963199535451837141,
+    -0.98982144188093273238};
+
+struct FarLDS {
+    double xn[kFN];                   // node positions [cm^-1]
+    double acc[kFN][kThreads / 64];   // node sums, one column per wave
+    double c[kFN];                    // Chebyshev coefficients
+    int cnt[kThreads / 64];
+    int any;
+};
+
+// S Re w(x, y) / sqrt(pi) with Bp = S y / pi, from the asymptotic series.
+__device__ __forceinline__ double far_node64(double x, double y) {
+    const double r2 = fma(x, x, y * y);
+    const double ir = rcp64(r2);
+    const double zr = x * ir, zi = -y * ir;               // 1/z
+    const double ur = (zr - zi) * (zr + zi), ui = 2.0 * zr * zi;   // 1/z^2
+    constexpr double c6 = 10395.0 / 64.0, c5 = 945.0 / 32.0, c4 = 105.0 / 16.0,
+                     c3 = 15.0 / 8.0, c2 = 3.0 / 4.0, c1 = 0.5;
+    double sr = c6, si = 0.0;
+#pragma unroll
+    for (int n = 5; n >= 0; --n) {
+        const double cn = n == 5 ? c5 : n == 4 ? c4 : n == 3 ? c3 : n == 2 ? c2 : n == 1 ? c1 : 1.0;
+        const double tr = fma(sr, ur, fma(-si, ui, cn));
+        si = fma(sr, ui, si * ur);
+        sr = tr;
+    }
+    // Re(i S / z) = -(Im S Re(1/z) + Re S Im(1/z))
+    return -(si * zr + sr * zi) * (1.0 / kPi);
+}
+
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Whether a line (centre nu, window [wl, wh), 1/alpha_D ia, y) goes through
+// the far field of the tile [j0, j0 + Wv) with half width hW (points).
+__device__ __forceinline__ bool far_line(double nu, int64_t wl, int64_t wh, double ia, double y,
+                                         int64_t j0, int64_t Wv, double hW, double numin,
+                                         double dnu) {
+    if (Wv < kFarMinPts || wl > j0 || wh < j0 + Wv) return false;
+    const double dpts = fabs((nu - numin) / dnu - ((double)j0 + hW));
+    if (dpts < kFarRatio * hW) return false;
+    const double xnear = (dpts - hW) * dnu * ia;
+    return fma(xnear, xnear, y * y) >= kFarR2;
+}
+
 template <int PPT>
-__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec) {
+__device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec, FarLDS *fl) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -423,16 +515,34 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
 #pragma unroll
     for (int k = 0; k < PPT; ++k) { hi[k] = 0.0f; lo[k] = 0.0f; part[k] = 0.0f; }
     int since = 0;
+    // far field: valid points of the tile, half width, nodes
+    const int64_t Wv = min((int64_t)W, a.nx - j0);
+    const double hW = 0.5 * (double)(Wv - 1);
+    const bool ffok = a.far && Wv >= kFarMinPts;
+    if (t < kFN) fl->xn[t] = fma((double)j0 + hW * (1.0 + kFarU[t]), a.dnu, a.numin);
+    if (t < kFN * (kThreads / 64)) fl->acc[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0;
+    if (t == 0) fl->any = 0;
+    __syncthreads();
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec32 r;
         r.lo = 0; r.hi = 0; r.code = 0;
+        bool far = false;
+        double fy = 0.0, fnu = 0.0, fia = 0.0, fB = 0.0;
         if (i < ti.hi) {
             const double nu = a.nu[i];
             int64_t wl, wh;
             cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
             const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
-            if (clo < chi) {
+            if (ffok && wl <= j0 && wh >= j0 + Wv) {
+                const double ia = a.ia[i], y = a.y[i];
+                if (far_line(nu, wl, wh, ia, y, j0, Wv, hW, a.numin, a.dnu)) {
+                    far = true;
+                    fy = y; fnu = nu; fia = ia;
+                    fB = ldexp(a.S[i], -ti.e);
+                }
+            }
+            if (clo < chi && !far) {
                 const double ia = a.ia[i], y = a.y[i];
                 // nearest grid point to the centre (may lie off the grid)
                 const double jcd = rint((nu - a.numin) / a.dnu);
@@ -479,11 +589,22 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, r2max, kMidMax32, kMid2Max32);
             }
         }
+        // far lines: FP64 values at the nodes, summed per wave
+        if (__ballot(far) != 0ull) {
+#pragma unroll 1
+            for (int k = 0; k < kFN; ++k) {
+                const double v = far ? fB * far_node64((fl->xn[k] - fnu) * fia, fy) : 0.0;
+                const double sw = wave_sum64(v);
+                if (lane == 0) fl->acc[k][wave] += sw;
+            }
+            if (lane == 0) fl->any = 1;
+        }
         // the previous stage's readers of rec[] are done after this
         __syncthreads();
-        rec[t] = r;
+        int nst;
+        const int slot = compact_slot(r.code != 0, fl->cnt, nst);
+        if (r.code != 0) rec[slot] = r;
         __syncthreads();
-        const int nst = (int)min((int64_t)kStage, ti.hi - base);
         for (int l = 0; l < nst; ++l) {
             const Rec32 &q = rec[l];
             const int code = (__builtin_amdgcn_readfirstlane(q.code) >> (4 * wave)) & 15;
@@ -496,12 +617,53 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             }
         }
     }
+    // far field: node sums -> Chebyshev coefficients -> the tile's points
+    __syncthreads();
+    const bool any = fl->any != 0;
+    if (any) {
+        if (t < kFN) {
+            double F = 0.0;
+#pragma unroll
+            for (int w = 0; w < kThreads / 64; ++w) F += fl->acc[t][w];
+            fl->acc[t][0] = F;
+        }
+        __syncthreads();
+        if (t < kFN) {
+            // c_t = (2 - [t == 0]) / kFN sum_k F_k T_t(u_k), T_t by its recurrence
+            double c = 0.0;
+            for (int k = 0; k < kFN; ++k) {
+                const double u = kFarU[k];
+                double tm = 1.0, tc = u;
+                for (int m = 2; m <= t; ++m) {
+                    const double tn = fma(2.0 * u, tc, -tm);
+                    tm = tc;
+                    tc = tn;
+                }
+                c += fl->acc[k][0] * (t == 0 ? 1.0 : tc);
+            }
+            fl->c[t] = (t == 0 ? 1.0 : 2.0) / (double)kFN * c;
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         two_sum_acc(hi[k], lo[k], part[k]);
         const int64_t j = j0 + jr0 + 64 * k;
+        double ff = 0.0;
+        if (any) {
+            // Clenshaw at u = (jr - hW) / hW
+            const double u = ((double)(jr0 + 64 * k) - hW) / hW;
+            double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+            for (int m = kFN - 1; m >= 1; --m) {
+                const double b0 = fma(2.0 * u, b1, fl->c[m] - b2);
+                b2 = b1;
+                b1 = b0;
+            }
+            ff = fma(u, b1, fl->c[0] - b2);
+        }
         if (j < a.nx) {
-            const double sum = ldexp((double)hi[k] + (double)lo[k], ti.e);
+            const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
             a.K[j] = a.accumulate ? a.K[j] + sum : a.kmin + sum;
         }
     }
@@ -517,25 +679,6 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
 // pairs.
 // ---------------------------------------------------------------------------
 constexpr int kCoreW = 256;
-
-// Position of this thread's valid record among the workgroup's valid
-// records (line order) and their total; cnt[] is 4 ints of LDS.
-__device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long bal = __ballot(valid);
-    const int below = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) cnt[wave] = __popcll(bal);
-    __syncthreads();
-    int off = 0;
-    total = 0;
-#pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) {
-        const int c = cnt[w];
-        off += w < wave ? c : 0;
-        total += c;
-    }
-    return off + below;
-}
 
 __device__ __forceinline__ void core_f64(const VArgs &a, int64_t tile, Core64 *rec, int *cnt) {
     constexpr int W = kCoreW;
@@ -687,7 +830,8 @@ __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
         tile_f64<PPT>(a, tile, rec);
     } else {
         __shared__ Rec32 rec[kStage];
-        tile_f32c<PPT>(a, tile, rec);
+        __shared__ FarLDS fl;
+        tile_f32c<PPT>(a, tile, rec, &fl);
     }
 }
 
@@ -719,14 +863,20 @@ __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__re
     __shared__ int cnt[kThreads / 64];
     __shared__ float dtab[(PPT == 0 && MODE == HK_VOIGT_FP32C) ? kStage * kDT : 1];
     __shared__ __attribute__((aligned(16))) char smem[kRecBytes * kStage];
+    __shared__ FarLDS fl[1];
     const int64_t total = item_start[nspec];
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1ull);
         __syncthreads();
-        const int64_t item = s_item;
+        // uniform across the workgroup: keep the item, spectrum and its
+        // arguments in scalar registers
+        const int64_t iv = s_item;
+        const int64_t item =
+            (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(iv >> 32)) << 32) |
+                      (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)iv));
         if (item >= total) break;
-        const int s = item_spectrum(item_start, nspec, item);
+        const int s = __builtin_amdgcn_readfirstlane(item_spectrum(item_start, nspec, item));
         const VArgs a = specs[s];
         const int64_t tile = item - item_start[s];
         if constexpr (PPT == 0) {
@@ -734,7 +884,7 @@ __global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__re
             else core_f32c(a, tile, (Core32 *)smem, dtab, cnt);
         } else {
             if constexpr (MODE == HK_VOIGT_FP64) tile_f64<PPT>(a, tile, (Rec64 *)smem);
-            else tile_f32c<PPT>(a, tile, (Rec32 *)smem);
+            else tile_f32c<PPT>(a, tile, (Rec32 *)smem, fl);
         }
     }
 }
@@ -788,6 +938,63 @@ __global__ __launch_bounds__(256) void pair_count_kernel(const double *__restric
 
 __global__ void zero4_kernel(unsigned long long *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
+}
+
+// Work of one FP32 Voigt sum with the far field at tile width W: per line,
+// the tiles its window touches, split into far tiles (pairs summed through
+// the far field, node evaluations) and direct pairs by region.  out: direct
+// A, B, C pairs, far-field pairs, far (line, tile) count.
+__global__ __launch_bounds__(256) void work_count_kernel(const double *__restrict__ nu,
+                                                         const double *__restrict__ hw,
+                                                         const double *__restrict__ ia,
+                                                         const double *__restrict__ y, int64_t nl,
+                                                         double numin, double dnu, int64_t nx,
+                                                         int W, int far,
+                                                         unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long red[5][256];
+    const int t = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * 256 + t;
+    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0;
+    if (i < nl) {
+        int64_t wl, wh, bl = 0, bh = 0, cl = 0, ch = 0;
+        cut_window(nu[i], hw[i], numin, dnu, nx, wl, wh);
+        const double yy = y[i], y2 = yy * yy, a = ia[i];
+        if (wh > wl && a > 0.0) {
+            if (y2 < 1.0e6) cut_window(nu[i], fmin(hw[i], sqrt(1.0e6 - y2) / a), numin, dnu, nx, bl, bh);
+            if (y2 < 100.0) cut_window(nu[i], fmin(hw[i], sqrt(100.0 - y2) / a), numin, dnu, nx, cl, ch);
+            auto ov = [](int64_t l0, int64_t h0, int64_t l1, int64_t h1) {
+                const int64_t l = max(l0, l1), h = min(h0, h1);
+                return (unsigned long long)(h > l ? h - l : 0);
+            };
+            for (int64_t tl = wl / W; tl <= (wh - 1) / W; ++tl) {
+                const int64_t j0 = tl * W, Wv = min((int64_t)W, nx - j0);
+                const double hW = 0.5 * (double)(Wv - 1);
+                if (far && far_line(nu[i], wl, wh, a, yy, j0, Wv, hW, numin, dnu)) {
+                    fp += (unsigned long long)Wv;
+                    fl += 1;
+                    continue;
+                }
+                const unsigned long long tot = ov(wl, wh, j0, j0 + Wv);
+                const unsigned long long bc = ov(bl, bh, j0, j0 + Wv);
+                const unsigned long long c = ov(cl, ch, j0, j0 + Wv);
+                cA += tot - bc;
+                cB += bc - c;
+                cC += c;
+            }
+        }
+    }
+    red[0][t] = cA; red[1][t] = cB; red[2][t] = cC; red[3][t] = fp; red[4][t] = fl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + s];
+        __syncthreads();
+    }
+    if (t < 5) atomicAdd(&out[t], red[t][0]);
+}
+
+__global__ void zero8_kernel(unsigned long long *c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0ull;
 }
 
 }  // namespace hk
@@ -887,6 +1094,7 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
     a.key = st->key; a.blk_max = st->blk_max; a.radius = st->radius; a.nl = st->nlines;
     a.tiles = tiles; a.numin = grid->numin; a.dnu = grid->dnu; a.nx = grid->nx;
     a.kmin = kmin; a.accumulate = accumulate; a.K = K;
+    a.far = (mode == HK_VOIGT_FP32C && !(tune && tune->exact_pairs)) ? 1 : 0;
     if (mode == HK_VOIGT_FP64) launch_tiles<HK_VOIGT_FP64>(ppt, ntiles, s, a);
     else launch_tiles<HK_VOIGT_FP32C>(ppt, ntiles, s, a);
     HK_LAUNCH_CHECK();
@@ -995,6 +1203,7 @@ extern "C" int hk_voigt_batched_prepare(int nspec, const hk_spectrum *spec, int 
         a.tiles = (TileInfo *)(base + L.tile_off[i]);
         a.numin = sp.grid.numin; a.dnu = sp.grid.dnu; a.nx = sp.grid.nx;
         a.kmin = sp.kmin; a.accumulate = sp.accumulate; a.K = sp.K;
+        a.far = (mode == HK_VOIGT_FP32C && !(tune && tune->exact_pairs)) ? 1 : 0;
         cargs[q] = a;
         cargs[q].radius = sp.st.radius + 1;
         cargs[q].tiles = (TileInfo *)(base + L.ctile_off[i]);
@@ -1084,8 +1293,9 @@ extern "C" int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, i
     HK_HIP(hipEventCreate(&e1));
     float bestms = 0.0f;
     int bestw = 0, rc = HK_OK;
+    const int exact = best->exact_pairs;     // tuned for the caller's setting
     for (int c = 0; c < 4 && rc == HK_OK; ++c) {
-        hk_tune t = {kCand[c], 0};
+        hk_tune t = {kCand[c], exact};
         rc = hk_voigt_accumulate(st, grid, 0.0, 0, mode, &t, K, ws, ws_bytes, stream);
         if (rc != HK_OK) break;
         if (hipEventRecord(e0, s) != hipSuccess) { rc = HK_ERR_HIP; break; }
@@ -1103,8 +1313,31 @@ extern "C" int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, i
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc == HK_OK) { best->tile_points = bestw; best->lines_per_stage = kStage; }
+    if (rc == HK_OK) best->tile_points = bestw;
     return rc;
+}
+
+extern "C" int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
+                                   const hk_tune *tune, unsigned long long *counts,
+                                   hk_stream_t stream) {
+    HK_CHECK_ARG(st && grid && counts, "hk_voigt_work_count: null argument");
+    HK_CHECK_ARG(mode == HK_VOIGT_FP64 || mode == HK_VOIGT_FP32C, "hk_voigt_work_count: mode %d",
+                 mode);
+    if (int e = check_state(st, "hk_voigt_work_count")) return e;
+    if (int e = check_grid(grid, "hk_voigt_work_count")) return e;
+    const int ppt = pick_ppt(mode, tune);
+    HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_work_count: tile_points must be 512, 1024, 2048 or 4096");
+    const int far = (mode == HK_VOIGT_FP32C && !(tune && tune->exact_pairs)) ? 1 : 0;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero8_kernel, dim3(1), dim3(64), 0, s, counts);
+    HK_LAUNCH_CHECK();
+    if (st->nlines > 0) {
+        hipLaunchKernelGGL(work_count_kernel, dim3((unsigned)((st->nlines + 255) / 256)), dim3(256),
+                           0, s, st->nu, st->hw, st->ialphaD, st->y, st->nlines, grid->numin,
+                           grid->dnu, grid->nx, kThreads * ppt, far, counts);
+        HK_LAUNCH_CHECK();
+    }
+    return HK_OK;
 }
 
 extern "C" int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,

@@ -185,14 +185,16 @@ def line_params(species, T, P, Q=None, cut_mode=0, cut=0.0, state=None, stream=N
     return st
 
 
-def _tune(tile_points=0):
-    return Tune(int(tile_points), 0)
+def _tune(tile_points=0, exact_pairs=False):
+    return Tune(int(tile_points), int(bool(exact_pairs)))
 
 
 def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, accumulate=False,
-          stream=None, workspace=None):
-    """K_j = kmin + sum_i S_i Re w / sqrt(pi) on the grid (or K += sum)."""
-    tune = _tune(tile_points)
+          stream=None, workspace=None, exact_pairs=False):
+    """K_j = kmin + sum_i S_i Re w / sqrt(pi) on the grid (or K += sum).
+    exact_pairs: FP32C evaluates every pair instead of interpolating the far
+    field of distant lines."""
+    tune = _tune(tile_points, exact_pairs)
     if K is None:
         K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
     nbytes = lib().hk_voigt_workspace_bytes(ctypes.byref(grid), ctypes.byref(tune))
@@ -223,6 +225,16 @@ def profile(state, grid, profile, remove_plinth=False, kmin=0.0, voigt_mode=HK_V
                                       int(voigt_mode), None if c is None else ctypes.byref(c),
                                       _ptr(K), _ptr(ws), nbytes, _stream(stream)))
     return K
+
+
+def work_count(state, grid, mode=HK_VOIGT_FP32C, tile_points=0, exact_pairs=False, stream=None):
+    """(direct A, B, C pairs, far-field pairs, far (line, tile) pairs) of
+    one Voigt sum with this form and tiling."""
+    c = torch.zeros(8, dtype=torch.int64, device=_dev())
+    tune = _tune(tile_points, exact_pairs)
+    check(lib().hk_voigt_work_count(ctypes.byref(state.st), ctypes.byref(grid), int(mode),
+                                    ctypes.byref(tune), _ptr(c), _stream(stream)))
+    return [int(v) for v in c.cpu().tolist()[:5]]
 
 
 def pair_count(state, grid, stream=None):

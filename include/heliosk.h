@@ -137,7 +137,9 @@ typedef struct {
 /* Launch tuning; zero fields select the defaults. */
 typedef struct {
     int tile_points;    /* grid points per workgroup tile */
-    int lines_per_stage;/* lines staged through LDS per step */
+    int exact_pairs;    /* HK_VOIGT_FP32C: 1 evaluates every (line, point)
+                           pair; 0 (default) sums lines far from a tile
+                           through its Chebyshev far field */
 } hk_tune;
 
 const char *hk_last_error(void);
@@ -185,6 +187,15 @@ int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
 int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
                         unsigned long long *counts, hk_stream_t stream);
 
+/* Work actually done by one hk_voigt_accumulate call with this mode and
+ * tuning (tile width set by tile_points): counts (8 device uint64) =
+ * [direct pairs in regions A, B, C (as hk_voigt_pair_count), pairs summed
+ * through the FP32 far field, far (line, tile) pairs -- each 11 node
+ * evaluations --, 0, 0, 0].  The direct and far-field pairs add up to
+ * counts[0] of hk_voigt_pair_count. */
+int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
+                        const hk_tune *tune, unsigned long long *counts, hk_stream_t stream);
+
 /* Autotuner (replaces the reference's launch self-tuning, heliosk.cu:1363-
  * 1401): times hk_voigt_accumulate on this input for each tile size in
  * {512, 1024, 2048, 4096} points (one untimed and one timed launch each,
@@ -192,7 +203,8 @@ int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
  * receives the timings.  K is overwritten; the workspace must be sized
  * for the smallest tile (hk_voigt_workspace_bytes covers every size).
  * Waits for the stream.  Results of different tile sizes agree to
- * rounding, not bit for bit. */
+ * rounding, not bit for bit.  best->exact_pairs is read: the sizes are
+ * timed with that setting. */
 int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, int mode,
                       double *K, void *workspace, size_t workspace_bytes,
                       hk_tune *best, float *ms, hk_stream_t stream);

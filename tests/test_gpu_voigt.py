@@ -303,3 +303,36 @@ def test_device_line_sort_is_stable(E, n):
         np.testing.assert_array_equal(getattr(sp, k).cpu().numpy(), v[o], err_msg=k)
     np.testing.assert_array_equal(sp.iso.cpu().numpy(), iso[o])
     assert sp.form == E.HK_FORM_EXOMOL and sp.nlines == n
+
+
+@pytest.mark.parametrize("cm, cut, dnu", [(0, 25.0, 0.001), (1, 400.0, 0.002), (0, 0.0, 0.004)])
+def test_fp32c_far_field_vs_exact_pairs(E, cm, cut, dnu):
+    """The FP32 form's far-field interpolation against the same form with
+    every pair evaluated and against the FP64 sum: it must change the
+    result (it is engaged) by far less than the 1e-5 bound."""
+    d = random_lines(30000, seed=44, numin=900.0, numax=1100.0, gamma=(0.01, 0.1))
+    st = E.line_params(species_from(d), 1500.0, 1.0, Q=d["iso_Q"], cut_mode=cm, cut=cut)
+    grid = E.make_grid(950.0, 1050.0, dnu)
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    Kff = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C).cpu().numpy()
+    Kex = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, exact_pairs=True).cpu().numpy()
+    e_ff, e_ex, d_fe = rel_err(Kff, K64), rel_err(Kex, K64), rel_err(Kff, Kex)
+    print("cut mode %d cut %g: far field vs FP64 %.3e, exact pairs vs FP64 %.3e, "
+          "between %.3e" % (cm, cut, e_ff, e_ex, d_fe))
+    assert e_ff <= 1e-5 and e_ex <= 1e-5
+    assert 0.0 < d_fe <= 2e-6
+    # deterministic
+    assert np.array_equal(Kff, E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C).cpu().numpy())
+
+
+def test_work_count_adds_up(E):
+    """Direct pairs by region plus far-field pairs equal the pair count; with
+    every pair evaluated (or in FP64) nothing goes through the far field."""
+    d = random_lines(20000, seed=46, numin=900.0, numax=1100.0)
+    st = E.line_params(species_from(d), 1800.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=25.0)
+    grid = E.make_grid(950.0, 1050.0, 0.001)
+    tot, A, B, C = E.pair_count(st, grid)
+    w = E.work_count(st, grid, E.HK_VOIGT_FP32C)
+    assert w[3] > 0 and w[4] > 0 and sum(w[:4]) == tot
+    for kw in (dict(mode=E.HK_VOIGT_FP32C, exact_pairs=True), dict(mode=E.HK_VOIGT_FP64)):
+        assert E.work_count(st, grid, **kw) == [A, B, C, 0, 0]
