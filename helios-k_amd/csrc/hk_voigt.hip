@@ -852,8 +852,12 @@ __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
 // Persistent form: workgroups take (spectrum, tile) items from a queue.
 // item_start[s] = first item of spectrum s (nspec+1 entries).  PPT == 0
 // selects the line-core pass.
+// Occupancy floor for the main pass (4 waves per SIMD for tiles up to 1024
+// points, 3 for 2048): without it the far-field code raises the register
+// count of the persistent form enough to halve its occupancy.
 template <int MODE, int PPT>
-__global__ __launch_bounds__(256) void voigt_persistent_kernel(const VArgs *__restrict__ specs,
+__global__ __launch_bounds__(256, (PPT != 0 && PPT <= 4) ? 4 : (PPT == 8 ? 3 : 1))
+void voigt_persistent_kernel(const VArgs *__restrict__ specs,
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
     constexpr size_t kRecBytes =
