@@ -43,8 +43,8 @@ FLOPS_PER_POINT_FAR = 25.0
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
 PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
 PEAK_HBM_GBS = 8000.0
-# The two launches of one FP32C Voigt sum at the default 2048-point tiles.
-VOIGT_KERNELS = ("void hk::voigt_tiles_kernel<1, 8>(hk::VArgs)",
+# The two launches of one FP32C Voigt sum at the default 1024-point tiles.
+VOIGT_KERNELS = ("void hk::voigt_tiles_kernel<1, 4>(hk::VArgs)",
                  "void hk::voigt_core_kernel<1>(hk::VArgs)")
 
 

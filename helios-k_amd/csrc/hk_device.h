@@ -215,35 +215,60 @@ __device__ __forceinline__ double rew_core64(double x, double y, double ey, doub
 
 // FP32 core for r2 < 25: n0 = ceil(x/a) <= 7, so every series term uses
 // n <= kNT32; dtab[n] = 1/(a^2 n^2 + y^2) is filled once per line and tile.
+// As in the FP64 core, each series is built from two hardware exponentials by
+// the ratio recurrences (4 exponentials per pair instead of 19); the
+// recurrences add < 7 roundings per term (< 5e-7 relative).
 constexpr int kNT32 = 13;
+constexpr float kEA32 = 5.559463031e-01f;       // e^{-a^2},  a = kA32
+constexpr float kE2A32 = 3.090762919e-01f;      // e^{-2a^2}
+constexpr float kE3A32 = 1.718298218e-01f;      // e^{-3a^2}
 
 __device__ __forceinline__ float rew_core32(float x, float y, float ey, float q1,
                                             const float *dtab) {
     constexpr float a = (float)kA32;
     x = fabsf(x);
     if (x == 0.0f) return ey;
-    const float xx = x * x, ex2 = expf(-xx);
+    const float xx = x * x, ex2 = __expf(-xx);
     if (y == 0.0f) return ex2;
-    float s2 = 0.0f, s3 = 0.0f;
+    // s2 = sum_n e^{-(an+x)^2} d_n: the n-th term times e^{-a^2(2n+1)} e^{-2ax}
+    // gives the next
+    const float t = __expf(-2.0f * a * x);
+    float tn = ex2 * kEA32 * t, g = kE3A32 * t, s2 = 0.0f;
 #pragma unroll
     for (int n = 1; n <= kN32; ++n) {
-        const float e = a * (float)n + x;
-        s2 = fmaf(__expf(-e * e), dtab[n], s2);
+        s2 = fmaf(tn, dtab[n], s2);
+        tn *= g;
+        g *= kE2A32;
     }
-    const int n0 = (int)ceilf(x / a);
+    // s3 = sum_{n>=1} e^{-(an-x)^2} d_n, from its largest term n0 = ceil(x/a)
+    // upwards and downwards
+    const int n0 = (int)ceilf(x * (float)(1.0 / kA32));
+    const float dl = fmaf(a, (float)n0, -x);
+    const float E0 = __expf(-dl * dl), e2 = __expf(-2.0f * a * dl);
+    float s3 = 0.0f;
+    {
+        float E = E0, rho = e2 * kEA32;
 #pragma unroll
-    for (int k = -kN32; k <= kN32; ++k) {
-        const int n = n0 + k;
-        const float d = a * (float)n - x;
-        const int nn = min(max(n, 1), kNT32);
-        const float tm = __expf(-d * d) * dtab[nn];
-        s3 += n >= 1 ? tm : 0.0f;
+        for (int k = 0; k <= kN32; ++k) {
+            s3 = fmaf(E, dtab[n0 + k], s3);
+            E *= rho;
+            rho *= kE2A32;
+        }
+    }
+    {
+        float E = E0, sg = kEA32 * rcp32(e2);
+#pragma unroll
+        for (int k = 1; k <= kN32; ++k) {
+            E *= sg;
+            sg *= kE2A32;
+            s3 += n0 - k >= 1 ? E * dtab[max(n0 - k, 1)] : 0.0f;
+        }
     }
     const float xy = x * y;
     float sxy, cxy;
     sincosf(xy, &sxy, &cxy);
     const float c2 = fmaf(cxy, cxy, -sxy * sxy);
-    const float sxy2_over_y = sxy * x * (xy < 1e-4f ? 1.0f : sxy / xy);
+    const float sxy2_over_y = sxy * x * (xy < 1e-4f ? 1.0f : sxy * rcp32(xy));
     constexpr float ca = (float)(2.0 * kA32 / kPi);
     return ex2 * ey * c2 + ca * (sxy2_over_y * ex2 + y * fmaf(-c2, ex2 * q1, 0.5f * (s2 + s3)));
 }

@@ -1008,7 +1008,10 @@ using namespace hk;
 namespace {
 
 int pick_ppt(int mode, const hk_tune *tune) {
-    int tp = tune && tune->tile_points > 0 ? tune->tile_points : (mode == HK_VOIGT_FP64 ? 1024 : 2048);
+    // defaults: 1024-point tiles for both forms (with the FP32 far field,
+    // smaller tiles trade direct pairs for node evaluations; 1024 was
+    // fastest at 1e7 lines x 3e7 points)
+    int tp = tune && tune->tile_points > 0 ? tune->tile_points : 1024;
     return tp / kThreads;
 }
 
