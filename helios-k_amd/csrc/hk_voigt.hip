@@ -489,15 +489,23 @@ __device__ __forceinline__ double wave_sum64(double v) {
     return v;
 }
 
-// Whether a line (centre nu, window [wl, wh), 1/alpha_D ia, y) goes through
-// the far field of the tile [j0, j0 + Wv) with half width hW (points).
-__device__ __forceinline__ bool far_line(double nu, int64_t wl, int64_t wh, double ia, double y,
-                                         int64_t j0, int64_t Wv, double hW, double numin,
-                                         double dnu) {
-    if (Wv < kFarMinPts || wl > j0 || wh < j0 + Wv) return false;
-    const double dpts = fabs((nu - numin) / dnu - ((double)j0 + hW));
-    if (dpts < kFarRatio * hW) return false;
-    const double xnear = (dpts - hW) * dnu * ia;
+// Whether a line (centre nu, cut half width hw, 1/alpha_D ia, y) goes
+// through the far field of the tile [j0, j0 + Wv) with half width hW
+// (points): its window covers the tile (the cut predicate |x_j - nu| < hw
+// holds at both end points, so at all of them), its centre is at least
+// kFarRatio half widths from the tile's centre and x^2 + y^2 >= kFarR2 at
+// the nearest point.  No division, so it is cheap enough to run for every
+// staged line before its window is computed.
+__device__ __forceinline__ bool far_line(double nu, double hw, double ia, double y, int64_t j0,
+                                         int64_t Wv, double hW, double numin, double dnu) {
+    if (Wv < kFarMinPts || !(hw > 0.0)) return false;
+    const double dlo = fma((double)j0, dnu, numin) - nu;
+    const double dhi = fma((double)(j0 + Wv - 1), dnu, numin) - nu;
+    if (!(dlo > -hw && dlo < hw && dhi > -hw && dhi < hw)) return false;
+    const double half = hW * dnu;
+    const double d = fabs(nu - fma((double)j0 + hW, dnu, numin));
+    if (d < kFarRatio * half) return false;
+    const double xnear = (d - half) * ia;
     return fma(xnear, xnear, y * y) >= kFarR2;
 }
 
@@ -528,21 +536,33 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         Rec32 r;
         r.lo = 0; r.hi = 0; r.code = 0;
         bool far = false;
-        double fy = 0.0, fnu = 0.0, fia = 0.0, fB = 0.0;
-        if (i < ti.hi) {
+        // far lines first: FP64 values at the nodes, summed per wave (their
+        // registers are free again before the direct records are built)
+        if (ffok) {
+            double fy = 0.0, fnu = 0.0, fia = 0.0, fB = 0.0;
+            if (i < ti.hi) {
+                fnu = a.nu[i];
+                fia = a.ia[i];
+                fy = a.y[i];
+                far = far_line(fnu, a.hw[i], fia, fy, j0, Wv, hW, a.numin, a.dnu);
+                if (far) fB = ldexp(a.S[i], -ti.e);
+            }
+            if (__ballot(far) != 0ull) {
+#pragma unroll 1
+                for (int k = 0; k < kFN; ++k) {
+                    const double v = far ? fB * far_node64((fl->xn[k] - fnu) * fia, fy) : 0.0;
+                    const double sw = wave_sum64(v);
+                    if (lane == 0) fl->acc[k][wave] += sw;
+                }
+                if (lane == 0) fl->any = 1;
+            }
+        }
+        if (i < ti.hi && !far) {
             const double nu = a.nu[i];
             int64_t wl, wh;
             cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
             const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
-            if (ffok && wl <= j0 && wh >= j0 + Wv) {
-                const double ia = a.ia[i], y = a.y[i];
-                if (far_line(nu, wl, wh, ia, y, j0, Wv, hW, a.numin, a.dnu)) {
-                    far = true;
-                    fy = y; fnu = nu; fia = ia;
-                    fB = ldexp(a.S[i], -ti.e);
-                }
-            }
-            if (clo < chi && !far) {
+            if (clo < chi) {
                 const double ia = a.ia[i], y = a.y[i];
                 // nearest grid point to the centre (may lie off the grid)
                 const double jcd = rint((nu - a.numin) / a.dnu);
@@ -588,16 +608,6 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 };
                 r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, r2max, kMidMax32, kMid2Max32);
             }
-        }
-        // far lines: FP64 values at the nodes, summed per wave
-        if (__ballot(far) != 0ull) {
-#pragma unroll 1
-            for (int k = 0; k < kFN; ++k) {
-                const double v = far ? fB * far_node64((fl->xn[k] - fnu) * fia, fy) : 0.0;
-                const double sw = wave_sum64(v);
-                if (lane == 0) fl->acc[k][wave] += sw;
-            }
-            if (lane == 0) fl->any = 1;
         }
         // the previous stage's readers of rec[] are done after this
         __syncthreads();
@@ -973,7 +983,7 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
             for (int64_t tl = wl / W; tl <= (wh - 1) / W; ++tl) {
                 const int64_t j0 = tl * W, Wv = min((int64_t)W, nx - j0);
                 const double hW = 0.5 * (double)(Wv - 1);
-                if (far && far_line(nu[i], wl, wh, a, yy, j0, Wv, hW, numin, dnu)) {
+                if (far && far_line(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu)) {
                     fp += (unsigned long long)Wv;
                     fl += 1;
                     continue;
