@@ -98,6 +98,18 @@ struct Core32 {
     int32_t d0, lo, hi, pad;
 };
 
+// Main-pass record loops: lane l of a wave reads the dispatch nibble of
+// record c0 + l, one ballot marks the records this wave must take, and the
+// wave walks the set bits in ascending order with scalar instructions, so
+// records that miss the wave cost nothing and no record waits on an LDS read
+// of its own code.  (The line-core passes keep a plain loop: the ballot form
+// costs them a register block and a wave of occupancy.)
+__device__ __forceinline__ int lowest_bit(unsigned long long &m) {
+    const int b = __builtin_ctzll(m);
+    m &= m - 1ull;
+    return b;
+}
+
 // Spectrum of a queue item: item_start[s] <= item < item_start[s + 1].
 __device__ __forceinline__ int item_spectrum(const int64_t *__restrict__ item_start, int nspec,
                                              int64_t item) {
@@ -306,11 +318,14 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
         rec[t] = r;
         __syncthreads();
         const int nst = (int)min((int64_t)kStage, ti.hi - base);
-        for (int l = 0; l < nst; ++l) {
-            const Rec64 &q = rec[l];
-            const int code = (__builtin_amdgcn_readfirstlane(q.code) >> (4 * wave)) & 15;
-            if (code == 0) continue;                         // window misses this wave
-            dispatch64<PPT>(q, code, xj, acc, jr0);
+        for (int c0 = 0; c0 < nst; c0 += 64) {
+            const int nib = c0 + lane < nst ? (rec[c0 + lane].code >> (4 * wave)) & 15 : 0;
+            unsigned long long m = __ballot(nib != 0);       // 0: window misses this wave
+            while (m) {
+                const int b = lowest_bit(m);
+                const int code = __builtin_amdgcn_readlane(nib, b);
+                dispatch64<PPT>(rec[c0 + b], code, xj, acc, jr0);
+            }
         }
     }
 #pragma unroll
@@ -615,15 +630,18 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         const int slot = compact_slot(r.code != 0, fl->cnt, nst);
         if (r.code != 0) rec[slot] = r;
         __syncthreads();
-        for (int l = 0; l < nst; ++l) {
-            const Rec32 &q = rec[l];
-            const int code = (__builtin_amdgcn_readfirstlane(q.code) >> (4 * wave)) & 15;
-            if (code == 0) continue;                         // window misses this wave
-            dispatch32<PPT>(q, code, part, jr0);
-            if (++since == 32) {
+        for (int c0 = 0; c0 < nst; c0 += 64) {
+            const int nib = c0 + lane < nst ? (rec[c0 + lane].code >> (4 * wave)) & 15 : 0;
+            unsigned long long m = __ballot(nib != 0);       // 0: window misses this wave
+            while (m) {
+                const int b = lowest_bit(m);
+                const int code = __builtin_amdgcn_readlane(nib, b);
+                dispatch32<PPT>(rec[c0 + b], code, part, jr0);
+                if (++since == 32) {
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
-                since = 0;
+                    for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
+                    since = 0;
+                }
             }
         }
     }
@@ -862,11 +880,12 @@ __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
 // Persistent form: workgroups take (spectrum, tile) items from a queue.
 // item_start[s] = first item of spectrum s (nspec+1 entries).  PPT == 0
 // selects the line-core pass.
-// Occupancy floor for the main pass (4 waves per SIMD for tiles up to 1024
-// points, 3 for 2048): without it the far-field code raises the register
-// count of the persistent form enough to halve its occupancy.
+// Occupancy floors: the main pass 4 waves per SIMD for tiles up to 1024
+// points, 3 for 2048 (without them the far-field code raises the register
+// count of the persistent form enough to halve its occupancy), the FP32
+// line-core pass 5 (as its one-spectrum kernel has without a floor).
 template <int MODE, int PPT>
-__global__ __launch_bounds__(256, (PPT != 0 && PPT <= 4) ? 4 : (PPT == 8 ? 3 : 1))
+__global__ __launch_bounds__(256, PPT == 0 ? (MODE == HK_VOIGT_FP32C ? 5 : 1) : (PPT <= 4 ? 4 : (PPT == 8 ? 3 : 1)))
 void voigt_persistent_kernel(const VArgs *__restrict__ specs,
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
