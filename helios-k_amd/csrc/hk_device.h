@@ -63,15 +63,13 @@ __device__ __forceinline__ double rcp64(double x) {
 // Voigt regions.  r2 = x^2 + y^2.
 //  FP64: core r2 < 100 (Algorithm 916), 100..1e4 Gauss-Hermite 8 nodes,
 //        1e4..1e6 Gauss-Hermite 4 nodes, >= 1e6 three-term asymptotic.
-//  FP32: core r2 < 64 (Algorithm 916), 64..4096 Gauss-Hermite 4 nodes,
-//        >= 4096 two-term asymptotic.
 // Gauss-Hermite: Re w = (y/pi) sum_k w_k / ((x - t_k)^2 + y^2); each
 // symmetric pair folds to 2 w u / D with u = r2 + t^2, D = u^2 - 4 t^2 x^2.
 // --------------------------------------------------------------------------
 constexpr double kRB64 = 100.0, kRM64 = 1.0e4, kRF64 = 1.0e6;
 // FP32: core r2 < 25 (Algorithm 916), 25..100 8-node Gauss-Hermite plus the
-// Doppler term e^{-x^2} for near-Doppler lines (y < kY0), 100..4096 4-node
-// Gauss-Hermite, >= 4096 two-term asymptotic.
+// Doppler term e^{-x^2} for near-Doppler lines (y < kY0), 100..4096 the
+// four-term asymptotic series, >= 4096 the two-term one (far4_32, far32).
 constexpr float kRB32 = 25.0f, kRM32 = 100.0f, kRF32 = 4096.0f;
 constexpr float kY0 = 0.1f;
 
@@ -108,15 +106,6 @@ __device__ __forceinline__ double gh8_sum64(double x2, double r2) {
     return num * rcp64(D12 * D34);
 }
 
-__device__ __forceinline__ float gh4_sum32(float x2, float r2) {
-    const float a1 = (float)(kGH4t1 * kGH4t1), a2 = (float)(kGH4t2 * kGH4t2);
-    float u1 = r2 + a1, u2 = r2 + a2;
-    float D1 = fmaf(u1, u1, -4.0f * a1 * x2);
-    float D2 = fmaf(u2, u2, -4.0f * a2 * x2);
-    float num = fmaf((float)kGH4w1 * u1, D2, (float)kGH4w2 * u2 * D1);
-    return num * rcp32(D1 * D2);
-}
-
 __device__ __forceinline__ float gh8_sum32(float x2, float r2) {
     const float a1 = (float)(kGH8t1 * kGH8t1), a2 = (float)(kGH8t2 * kGH8t2);
     const float a3 = (float)(kGH8t3 * kGH8t3), a4 = (float)(kGH8t4 * kGH8t4);
@@ -142,11 +131,11 @@ __device__ __forceinline__ float gh8_sum32(float x2, float r2) {
 // a = pi/sqrt(-ln(tol/2)).  y is constant along a line, so erfcx(y) and
 // q1 = sum_n e^{-a^2 n^2} d_n (s1 = e^{-x^2} q1) are computed once per line
 // and tile (core_line_consts*).  The series keep the terms whose exponent is
-// above -(a N)^2: FP64 tol 1e-15, N = 13 (e^-47); FP32 tol 1e-7, N = 6
-// (e^-21).  The FP64 core builds the exponentials of each series from two
-// exp() calls by the ratio recurrences e^{-(a(n+1)+x)^2} = e^{-(an+x)^2}
-// e^{-2ax} e^{-a^2(2n+1)}; the FP32 core calls the hardware exponential per
-// term.
+// above -(a N)^2: FP64 tol 1e-15, N = 13 (e^-47); FP32 tol 1e-7, N = 5
+// (e^-14.7; the first omitted term is below e^-21 = 6.7e-10 of the leading
+// one).  Both cores build the exponentials of each series from two exp()
+// calls by the ratio recurrences e^{-(a(n+1)+x)^2} = e^{-(an+x)^2} e^{-2ax}
+// e^{-a^2(2n+1)}.
 // --------------------------------------------------------------------------
 constexpr double kA64 = 0.529275382982092;          // tol 1e-15
 constexpr int kN64 = 13;
@@ -154,7 +143,7 @@ constexpr double kEA64 = 0.755683658862919;         // e^{-a^2}
 constexpr double kE2A64 = 0.5710577922724486;       // e^{-2a^2}
 constexpr double kE3A64 = 0.4315390418866247;       // e^{-3a^2}
 constexpr double kA32 = 0.7662137865078357;         // tol 1e-7
-constexpr int kN32 = 6;
+constexpr int kN32 = 5;
 
 __device__ __forceinline__ double core_q1(double y, double a, int N) {
     double s = 0.0, yy = y * y;
@@ -214,7 +203,9 @@ __device__ __forceinline__ double rew_core64(double x, double y, double ey, doub
 }
 
 // FP32 core for r2 < 25: n0 = ceil(x/a) <= 7, so every series term uses
-// n <= kNT32; dtab[n] = 1/(a^2 n^2 + y^2) is filled once per line and tile.
+// n <= kNT32; dtab[n] = 1/(a^2 n^2 + y^2) is filled once per line and tile,
+// with dtab[0] = 0 so that the downward terms below n = 1 vanish without a
+// branch.
 // As in the FP64 core, each series is built from two hardware exponentials by
 // the ratio recurrences (4 exponentials per pair instead of 19); the
 // recurrences add < 7 roundings per term (< 5e-7 relative).
@@ -241,7 +232,8 @@ __device__ __forceinline__ float rew_core32(float x, float y, float ey, float q1
         g *= kE2A32;
     }
     // s3 = sum_{n>=1} e^{-(an-x)^2} d_n, from its largest term n0 = ceil(x/a)
-    // upwards and downwards
+    // upwards (k <= kN32: exponents (ka + dl)^2, dl = a n0 - x in [0, a)) and
+    // downwards (k <= kN32 + 1: exponents (ka - dl)^2 fall one step later)
     const int n0 = (int)ceilf(x * (float)(1.0 / kA32));
     const float dl = fmaf(a, (float)n0, -x);
     const float E0 = __expf(-dl * dl), e2 = __expf(-2.0f * a * dl);
@@ -258,10 +250,10 @@ __device__ __forceinline__ float rew_core32(float x, float y, float ey, float q1
     {
         float E = E0, sg = kEA32 * rcp32(e2);
 #pragma unroll
-        for (int k = 1; k <= kN32; ++k) {
+        for (int k = 1; k <= kN32 + 1; ++k) {
             E *= sg;
             sg *= kE2A32;
-            s3 += n0 - k >= 1 ? E * dtab[max(n0 - k, 1)] : 0.0f;
+            s3 = fmaf(E, dtab[max(n0 - k, 0)], s3);
         }
     }
     const float xy = x * y;

@@ -95,7 +95,7 @@ struct Core64 {
 
 struct Core32 {
     float r, vbh, vbl, y, y2, A, ey, q1;
-    int32_t d0, lo, hi, pad;
+    int32_t lo, hi, d0, pad;
 };
 
 // Main-pass record loops: lane l of a wave reads the dispatch nibble of
@@ -755,7 +755,7 @@ __device__ __forceinline__ void core_f64(const VArgs &a, int64_t tile, Core64 *r
         __syncthreads();
         for (int l = 0; l < n; ++l) {
             const Core64 &q = rec[l];
-            if (q.hi <= wlo || q.lo >= whi) continue;
+            if ((q.hi <= wlo) | (q.lo >= whi)) continue;      // one LDS read for both
             touched = true;
             const double x = (xj - q.nu) * q.ia;
             const double r2 = fma(x, x, q.y2);
@@ -821,6 +821,7 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
         const int slot = compact_slot(valid, cnt, n);
         if (valid) {
             rec[slot] = r;
+            dtab[slot * kDT] = 0.0f;
             for (int k = 1; k <= kNT32; ++k) {
                 const double an = kA32 * k;
                 dtab[slot * kDT + k] = (float)(1.0 / fma(an, an, (double)r.y2));
@@ -829,7 +830,7 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
         __syncthreads();
         for (int l = 0; l < n; ++l) {
             const Core32 &q = rec[l];
-            if (q.hi <= wlo || q.lo >= whi) continue;
+            if ((q.hi <= wlo) | (q.lo >= whi)) continue;      // one LDS read for both
             touched = true;
             const float m = (float)(jr + q.d0);
             const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
