@@ -498,9 +498,29 @@ __device__ __forceinline__ double far_node64(double x, double y) {
     return -(si * zr + sr * zi) * (1.0 / kPi);
 }
 
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// DPP move of a double: each lane takes the value of the lane that the
+// (compile-time) DPP control selects; rows outside row_mask keep 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum of v over the wave, exact in lane 63 (other lanes hold partial sums),
+// in a fixed order: pairs and quads (quad_perm), 8 and 16 lanes (row half
+// mirror, row mirror), then rows 0+1 and 2+3 (row_bcast:15 into rows 1 and
+// 3) and all four (row_bcast:31 into rows 2 and 3).  DPP moves are VALU
+// operations; a butterfly of __shfl_xor goes through ds_bpermute instead and
+// waits on the LDS path at every step.
+__device__ __forceinline__ double wave_total64(double v) {
+    v += dpp64<0xb1, 0xf>(v);     // quad_perm [1,0,3,2]
+    v += dpp64<0x4e, 0xf>(v);     // quad_perm [2,3,0,1]
+    v += dpp64<0x141, 0xf>(v);    // row_half_mirror
+    v += dpp64<0x140, 0xf>(v);    // row_mirror
+    v += dpp64<0x142, 0xa>(v);    // row_bcast:15
+    v += dpp64<0x143, 0xc>(v);    // row_bcast:31
     return v;
 }
 
@@ -566,8 +586,8 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
 #pragma unroll 1
                 for (int k = 0; k < kFN; ++k) {
                     const double v = far ? fB * far_node64((fl->xn[k] - fnu) * fia, fy) : 0.0;
-                    const double sw = wave_sum64(v);
-                    if (lane == 0) fl->acc[k][wave] += sw;
+                    const double sw = wave_total64(v);
+                    if (lane == 63) fl->acc[k][wave] += sw;
                 }
                 if (lane == 0) fl->any = 1;
             }
