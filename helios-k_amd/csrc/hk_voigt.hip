@@ -474,7 +474,7 @@ struct FarLDS {
     double xn[kFN];                   // node positions [cm^-1]
     double acc[kFN][kThreads / 64];   // node sums, one column per wave
     double c[kFN];                    // Chebyshev coefficients
-    int cnt[kThreads / 64];
+    int cnt[2][kThreads / 64];        // compaction counts, alternate stages
     int any;
 };
 
@@ -566,6 +566,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     if (t < kFN * (kThreads / 64)) fl->acc[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0;
     if (t == 0) fl->any = 0;
     __syncthreads();
+    int nprev = 0, par = 0;
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
         const int64_t i = base + t;
         Rec32 r;
@@ -644,12 +645,19 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, r2max, kMidMax32, kMid2Max32);
             }
         }
-        // the previous stage's readers of rec[] are done after this
-        __syncthreads();
+        // Barriers: the previous stage's readers of rec[] must be done before
+        // it is written, and the writes before this stage reads it; a stage
+        // of far lines only (nst == 0) needs neither, only the one inside
+        // compact_slot.  The counts alternate between two buffers, so a wave
+        // that skips ahead cannot overwrite counts another wave has not read
+        // (it cannot pass the next compact_slot barrier before that wave).
+        if (nprev > 0) __syncthreads();
         int nst;
-        const int slot = compact_slot(r.code != 0, fl->cnt, nst);
+        const int slot = compact_slot(r.code != 0, fl->cnt[par], nst);
+        par ^= 1;
         if (r.code != 0) rec[slot] = r;
-        __syncthreads();
+        if (nst > 0) __syncthreads();
+        nprev = nst;
         for (int c0 = 0; c0 < nst; c0 += 64) {
             const int nib = c0 + lane < nst ? (rec[c0 + lane].code >> (4 * wave)) & 15 : 0;
             unsigned long long m = __ballot(nib != 0);       // 0: window misses this wave
