@@ -456,8 +456,9 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // point; each adds its FP64 values at the kFN nodes, and the sum over them is
 // interpolated to the tile's points once.  The node values use the
 // asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n z^-2n, c_n = (2n-1)!!/2^n,
-// n <= 6, which requires x^2 + y^2 >= kFarR2 over the tile (truncation
-// < 1e-14 relative).
+// n <= 3, which requires x^2 + y^2 >= kFarR2 over the tile (first omitted
+// term of the real part (105/16) 9 y / x^10 against y / x^2: < 3e-9
+// relative).
 // ---------------------------------------------------------------------------
 constexpr int kFN = 11;
 constexpr double kFarRatio = 4.0;
@@ -484,12 +485,11 @@ __device__ __forceinline__ double far_node64(double x, double y) {
     const double ir = rcp64(r2);
     const double zr = x * ir, zi = -y * ir;               // 1/z
     const double ur = (zr - zi) * (zr + zi), ui = 2.0 * zr * zi;   // 1/z^2
-    constexpr double c6 = 10395.0 / 64.0, c5 = 945.0 / 32.0, c4 = 105.0 / 16.0,
-                     c3 = 15.0 / 8.0, c2 = 3.0 / 4.0, c1 = 0.5;
-    double sr = c6, si = 0.0;
+    constexpr double c3 = 15.0 / 8.0, c2 = 3.0 / 4.0, c1 = 0.5;
+    double sr = c3, si = 0.0;
 #pragma unroll
-    for (int n = 5; n >= 0; --n) {
-        const double cn = n == 5 ? c5 : n == 4 ? c4 : n == 3 ? c3 : n == 2 ? c2 : n == 1 ? c1 : 1.0;
+    for (int n = 2; n >= 0; --n) {
+        const double cn = n == 2 ? c2 : n == 1 ? c1 : 1.0;
         const double tr = fma(sr, ur, fma(-si, ui, cn));
         si = fma(sr, ui, si * ur);
         sr = tr;
