@@ -85,8 +85,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one device copy of this many MiB first (HBM counter calibration)")
-    ap.add_argument("--cpu-lines", type=int, default=10000,
-                    help="lines of the CPU-baseline sample (first lines by generator index)")
+    ap.add_argument("--cpu-lines", type=int, default=100000,
+                    help="lines of the CPU-baseline sample (first lines by generator index; "
+                         "the default is about 10 s on 16 threads at the default workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "

@@ -30,4 +30,8 @@ void set_error(const char *fmt, ...);
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// HK_VOIGT_COMPAT sum (hk_compat.hip); ws holds hk_voigt_workspace_bytes.
+int compat_accumulate(const hk_line_state *st, const hk_grid *grid, double kmin, int accumulate,
+                      double *K, void *ws, hipStream_t s);
+
 }  // namespace hk

@@ -71,7 +71,7 @@ __device__ __forceinline__ void tile_range(const double *__restrict__ key, int64
 }
 
 namespace {
-__global__ __launch_bounds__(256) void tile_ranges_kernel(
+__global__ __launch_bounds__(256) __attribute__((unused)) void tile_ranges_kernel(
     const double *__restrict__ key, int64_t nl, const double *__restrict__ radius,
     const double *__restrict__ blk_max, double numin, double dnu, int64_t nx, int tile_pts,
     int64_t ntiles, int scale, TileInfo *__restrict__ tiles) {

@@ -1128,8 +1128,8 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
                                    int accumulate, int mode, const hk_tune *tune, double *K,
                                    void *ws, size_t ws_bytes, hk_stream_t stream) {
     HK_CHECK_ARG(st && grid && K && ws, "hk_voigt_accumulate: null argument");
-    HK_CHECK_ARG(mode == HK_VOIGT_FP64 || mode == HK_VOIGT_FP32C, "hk_voigt_accumulate: mode %d",
-                 mode);
+    HK_CHECK_ARG(mode == HK_VOIGT_FP64 || mode == HK_VOIGT_FP32C || mode == HK_VOIGT_COMPAT,
+                 "hk_voigt_accumulate: mode %d", mode);
     if (int e = check_state(st, "hk_voigt_accumulate")) return e;
     if (int e = check_grid(grid, "hk_voigt_accumulate")) return e;
     const int ppt = pick_ppt(mode, tune);
@@ -1139,6 +1139,8 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
                   hk_voigt_workspace_bytes(grid, tune));
         return HK_ERR_WORKSPACE;
     }
+    if (mode == HK_VOIGT_COMPAT)
+        return compat_accumulate(st, grid, kmin, accumulate, K, ws, (hipStream_t)stream);
     const int W = kThreads * ppt;
     const int64_t ntiles = (grid->nx + W - 1) / W;
     const int64_t ncore = (grid->nx + kCoreW - 1) / kCoreW;

@@ -33,8 +33,9 @@
 // (subLorentzianfile) are supported.
 //
 // Extensions (not in the reference): the optional param.dat key
-// "precision = 0|1" and console flag "-prec 0|1" select the plain FP64
-// Voigt form (0, default) or the FP32-compensated form (1); the console
+// "precision = 0|1|2" and console flag "-prec 0|1|2" select the plain FP64
+// Voigt form (0, default), the FP32-compensated form (1) or the reference's
+// own FP32 region formulas with FP64 sums (2, HK_VOIGT_COMPAT); the console
 // flag "-maxlines N" sets the lines per device chunk (default 2^26).
 #include <hip/hip_runtime.h>
 
@@ -828,7 +829,8 @@ bool add_species(const Param &p, const Molecule &m, const std::vector<double> &Q
         return L;
     };
     hk_cut c = {p.cutMode, p.cut};
-    const int mode = p.precision == 1 ? HK_VOIGT_FP32C : HK_VOIGT_FP64;
+    const int mode = p.precision == 1 ? HK_VOIGT_FP32C
+                     : (p.precision == 2 ? HK_VOIGT_COMPAT : HK_VOIGT_FP64);
     // removePlinth acts only on cut lines (heliosk.cu:1178)
     const int plinth = p.removePlinth == 1 && p.cut != 0.0;
 
@@ -964,7 +966,9 @@ bool write_info(FILE *f, const Param &p, int64_t Nx, const char *device) {
     std::fprintf(f, "profile = %d\n", p.profile);
     std::fprintf(f, "doTuning = %d\n", p.doTuning);
     std::fprintf(f, "precision = %d (%s)\n", p.precision,
-                 p.precision == 1 ? "FP32 profile, compensated sums" : "FP64");
+                 p.precision == 1 ? "FP32 profile, compensated sums"
+                 : (p.precision == 2 ? "FP32 region formulas of the reference, FP64 sums"
+                                     : "FP64"));
     if (p.edges != "-") std::fprintf(f, "use output edges: %s\n", p.edges.c_str());
     return true;
 }
@@ -1095,7 +1099,7 @@ bool run(int argc, char **argv) {
     const bool useSubL = p.subLorentzian != "-";
     hk_chi chi;
     if (useSubL && !read_sublorentz(p, chi)) return false;
-    if (p.precision != 0 && p.precision != 1) return fail("Error: precision must be 0 or 1");
+    if (p.precision < 0 || p.precision > 2) return fail("Error: precision must be 0, 1 or 2");
     if (p.maxlines < 1 || p.maxlines > 0x7fffffffLL)
         return fail("Error: maxlines must lie in 1 .. 2147483647");
     if (!(p.dnu > 0.0)) return fail("Error: dnu must be > 0");

@@ -13,6 +13,7 @@ HK_OK = 0
 HK_FORM_HITRAN, HK_FORM_EXOMOL, HK_FORM_SUPERLINE = 0, 1, 2
 HK_VOIGT_FP64 = 0
 HK_VOIGT_FP32C = 1
+HK_VOIGT_COMPAT = 2
 HK_PROFILE_VOIGT, HK_PROFILE_LORENTZ, HK_PROFILE_DOPPLER, HK_PROFILE_BINNED_GAUSS = 1, 2, 3, 4
 HK_MAX_ISO = 16
 

@@ -14,7 +14,7 @@ import torch
 from . import _lib
 from ._lib import (HK_FORM_EXOMOL, HK_FORM_HITRAN, HK_FORM_SUPERLINE, HK_PROFILE_BINNED_GAUSS,
                    HK_PROFILE_DOPPLER, HK_PROFILE_LORENTZ, HK_PROFILE_VOIGT, HK_VOIGT_FP32C,
-                   HK_VOIGT_FP64, HK_MAX_ISO, Cut, Grid, LineState, Lines, Spectrum, TP, Tune,
+                   HK_VOIGT_FP64, HK_VOIGT_COMPAT, HK_MAX_ISO, Cut, Grid, LineState, Lines, Spectrum, TP, Tune,
                    check, lib)
 
 NA = 6.0221412927e23          # define.h:42
@@ -22,7 +22,8 @@ NA = 6.0221412927e23          # define.h:42
 __all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
            "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
            "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
-           "ktable_reconstruct", "adjoint", "synth_lines", "means", "means_x", "sort_bins", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C"]
+           "ktable_reconstruct", "adjoint", "synth_lines", "means", "means_x", "sort_bins", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C",
+           "HK_VOIGT_COMPAT"]
 
 
 def _dev():

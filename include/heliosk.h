@@ -56,7 +56,12 @@ enum {
 /* Voigt evaluation forms. */
 enum {
     HK_VOIGT_FP64 = 0,          /* plain FP64 profile, FP64 accumulation */
-    HK_VOIGT_FP32C = 1          /* FP32 profile, compensated FP32 tile sums */
+    HK_VOIGT_FP32C = 1,         /* FP32 profile, compensated FP32 tile sums */
+    HK_VOIGT_COMPAT = 2         /* the reference's own FP32 region formulas
+                                   (Line6fA/B/C_kernel, voigt.h:1177-1706:
+                                   A S1/r2, B its rational approximation,
+                                   C Algorithm 916 summed to def_TOLF), FP64
+                                   accumulation; hk_voigt_accumulate only */
 };
 
 /* Line profiles (param.dat "profile", voigt.h:1235-1262). */
@@ -172,7 +177,12 @@ int hk_line_params(const hk_lines *lines, const hk_tp *tp, const hk_cut *cut,
 /* Voigt sum onto the grid: K_j = kmin + sum_i S_i Re w(x_ij, y_i)/sqrt(pi)
  * (accumulate == 0) or K_j += sum (accumulate != 0), over the lines whose
  * cut window holds j: |x_j - nu_i| < hw_i in FP64.  Deterministic: each
- * point is owned by one thread that adds the lines in index order. */
+ * point is owned by one thread that adds the lines in index order.
+ * HK_VOIGT_COMPAT instead evaluates each pair as the reference's FP32
+ * kernels do (x = va + j vb in float, kept when x^2 < vcut2, region formulas
+ * of Line6fA/B/C_kernel, S3_kernel's FP32 line constants) and adds the
+ * values in FP64; its region-B formula is within 1.4e-2 of the exact
+ * profile, as the reference's is. */
 size_t hk_voigt_workspace_bytes(const hk_grid *grid, const hk_tune *tune);
 int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
                         double kmin, int accumulate, int mode,
@@ -272,7 +282,8 @@ int hk_ktable_plan_host(int Nxb, int nC, double *P);
  * voigt.h:1091-1262; S1 as S3_kernel forms it, voigt.h:458-470).  In Doppler
  * units x = (x_j - nu)/alpha_D, per pair inside the cut:
  *   HK_PROFILE_VOIGT         S Re w(x, y)/sqrt(pi)  (hk_voigt_accumulate in
- *                            voigt_mode, HK_VOIGT_FP64 or HK_VOIGT_FP32C)
+ *                            voigt_mode: HK_VOIGT_FP64, HK_VOIGT_FP32C or
+ *                            HK_VOIGT_COMPAT)
  *   HK_PROFILE_LORENTZ       (S y/pi) / (x^2 + y^2)
  *   HK_PROFILE_DOPPLER       (S/sqrt(pi)) exp(-x^2)
  *   HK_PROFILE_BINNED_GAUSS  (S/vb) (erf(x + vb/2) - erf(x - vb/2))/2, vb = dnu/alpha_D

@@ -313,6 +313,149 @@ void hko_voigt_points(int64_t nl, const double *nu, const double *S,
 }
 
 /* ------------------------------------------------------------------ */
+/* The reference's single-precision Voigt arithmetic ("compat").       */
+/* S3_kernel (voigt.h:436-482) rounds the FP64 line state to FP32:     */
+/*   S1 = (float)(S y / pi), Sf = (float)S, yf = (float)y,             */
+/*   va = (float)((numin - nu) ialphaD), vb = (float)(dnu ialphaD),    */
+/*   vcut2 = (float)(hw^2 ialphaD^2)  (cut mode 0: cut^2 ialphaD^2);   */
+/* each pair: x = fmaf((float)j, vb, va), t1 = x^2, r2 = t1 + yf^2,    */
+/* kept when t1 < vcut2, and                                           */
+/*   r2 >= 1e6 (Line6fA_kernel, voigt.h:1177-1281):  S1 / r2           */
+/*   100 <= r2 < 1e6 (Line6fB_kernel, voigt.h:1437-1515): S1 t4,       */
+/*     t4 = (pi t2 (2 t2 + r2) - 12 x^2) / (3 r2 (pi t2^2 - 6 x^2)),    */
+/*     t2 = r2 + 1.5                                                   */
+/*   r2 < 100 (Line6fC_kernel, voigt.h:1616-1706; Sigmabf,             */
+/*     voigt.h:217-268): Algorithm 916 in FP32, each series summed     */
+/*     until all three change by less than def_TOLF = 2.48e-12f, with  */
+/*     a = pi/sqrt(-ln(TOLF/2)), b = 1/sqrt(pi), c = 2a/pi             */
+/*     (heliosk.cu:1656-1658), times Sf b;                             */
+/* values added in FP64 in line order.  erfcxf(yf) is hko_erfcx(yf)    */
+/* rounded to float.  Built with -ffp-contract=off, so every float     */
+/* operation rounds as written.                                        */
+/* ------------------------------------------------------------------ */
+#define HKO_TOLF 2.48e-12f
+#define HKO_PIF 3.14159265358979323846f
+
+typedef struct { float a, b, c; } hko_cabc;
+
+static hko_cabc hko_compat_consts(void)
+{
+    hko_cabc k;
+    double a = (double)(float)(M_PI * sqrt(-1.0 / log((double)HKO_TOLF * 0.5)));
+    k.a = (float)a;
+    k.b = (float)(1.0 / sqrt(M_PI));
+    k.c = (float)(2.0 * a / M_PI);
+    return k;
+}
+
+static void hko_sigmabf(float x, float y, float a, float *s1o, float *s2o, float *s3o)
+{
+    float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
+    float yy = y * y;
+    x = fabsf(x);
+    int n0 = (int)ceilf(x / a);
+    int st1 = 0, st2 = 0, st3 = 0;
+    for (int n = 1; n < 100; ++n) {
+        int n3p = n0 + n - 1, n3n = n0 - n;
+        float an = a * (float)n, an3p = a * (float)n3p, an3n = a * (float)n3n;
+        float f = 1.0f / (an * an + yy);
+        float f3p = 1.0f / (an3p * an3p + yy);
+        float f3n = 1.0f / (an3n * an3n + yy);
+        s1 += f * expf(-(an * an + x * x));
+        s2 += f * expf(-(an + x) * (an + x));
+        s3 += f3p * expf(-(an3p - x) * (an3p - x));
+        if (n3n >= 1) s3 += f3n * expf(-(an3n - x) * (an3n - x));
+        if (fabsf(s1 - o1) < HKO_TOLF) st1 = 1;
+        if (fabsf(s2 - o2) < HKO_TOLF) st2 = 1;
+        if (fabsf(s3 - o3) < HKO_TOLF) st3 = 1;
+        if (st1 && st2 && st3) break;
+        o1 = s1; o2 = s2; o3 = s3;
+    }
+    *s1o = s1; *s2o = s2; *s3o = s3;
+}
+
+/* One pair: v = the reference's K increment for strengths S1f, Sf. */
+static float hko_compat_pair(float S1, float Sf, float y, float x, float vcut2, float ey,
+                             hko_cabc k)
+{
+    float t1 = x * x;
+    float xxyy = t1 + y * y;
+    if (!(t1 < vcut2)) return 0.0f;
+    if (xxyy >= 1.0e6f) return S1 / xxyy;
+    if (xxyy >= 100.0f) {
+        float t6 = t1 * 6.0f;
+        float t2 = xxyy + 1.5f;
+        float t3 = HKO_PIF * t2;
+        float t4 = (t3 * (2.0f * t2 + xxyy) - 2.0f * t6) / (3.0f * xxyy * (t3 * t2 - t6));
+        return S1 * t4;
+    }
+    float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    float ex2 = expf(-x * x);
+    if (x != 0.0f && y != 0.0f) hko_sigmabf(x, y, k.a, &s1, &s2, &s3);
+    float xy = x * y;
+    float cos2xy = cosf(2.0f * xy);
+    float sinxy = sinf(xy);
+    float u1 = ex2 * ey * cos2xy;
+    float u2 = sinxy * ex2 * sinxy / y;
+    float u3 = y * (-cos2xy * s1 + 0.5f * (s2 + s3));
+    u1 += k.c * (u2 + u3);
+    if (x == 0.0f) u1 = ey;
+    if (y == 0.0f) u1 = ex2;
+    return Sf * u1 * k.b;
+}
+
+/* The compat profile Re w(x, y)/sqrt(pi) for a unit line (S = 1, no cut):
+ * the three region formulas as functions of (x, y) alone. */
+float hko_compat_profile(float x, float y)
+{
+    hko_cabc k = hko_compat_consts();
+    float S1 = (float)(1.0 * (double)y / M_PI);
+    float ey = (float)hko_erfcx((double)y);
+    return hko_compat_pair(S1, 1.0f, y, x, INFINITY, ey, k);
+}
+
+void hko_voigt_compat_points(int64_t nl, const double *nu, const double *S,
+                             const double *ialphaD, const double *y,
+                             const double *hw, double R,
+                             double numin, double dnu, int64_t Nx, double kmin,
+                             int64_t npts, const int64_t *jidx, double *K,
+                             int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    hko_cabc k = hko_compat_consts();
+    /* the FP32 cut test can admit a point a little beyond the FP64 window */
+    double Rs = R + 2.0 * dnu + 1e-6 * R;
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t p = 0; p < npts; ++p) {
+        int64_t j = jidx ? jidx[p] : p;
+        double xj = fma((double)j, dnu, numin);
+        int64_t i0 = lower_bound(nu, nl, xj - Rs);
+        int64_t i1 = lower_bound(nu, nl, xj + Rs);
+        if (i1 < nl) ++i1;
+        double sum = 0.0;
+        for (int64_t i = i0; i < i1; ++i) {
+            if (!(hw[i] > 0.0)) continue;
+            double ia = ialphaD[i];
+            float S1 = (float)(S[i] * y[i] / M_PI);
+            float Sf = (float)S[i];
+            float yf = (float)y[i];
+            float va = (float)((numin - nu[i]) * ia);
+            float vb = (float)(dnu * ia);
+            float vcut2 = (float)(hw[i] * hw[i] * ia * ia);
+            float x = fmaf((float)j, vb, va);
+            if (!(x * x < vcut2)) continue;
+            float ey = (float)hko_erfcx((double)yf);
+            sum += (double)hko_compat_pair(S1, Sf, yf, x, vcut2, ey, k);
+        }
+        K[p] = kmin + sum;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Line profiles 1-4 (voigt.h:1235-1262 with S1 of S3_kernel,          */
 /* voigt.h:458-470) and the plinth (Plinth_kernel, voigt.h:1091-1164):  */
 /* per pair, in Doppler units x = (x_j - nu) / alpha_D:                 */

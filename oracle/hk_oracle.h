@@ -59,6 +59,15 @@ void hko_voigt_points(int64_t nl, const double *nu, const double *S,
                       int64_t npts, const int64_t *jidx, double *K,
                       int nthreads);
 
+/* ---- the reference's own FP32 Voigt arithmetic (compat form) ---- */
+float hko_compat_profile(float x, float y);
+void hko_voigt_compat_points(int64_t nl, const double *nu, const double *S,
+                             const double *ialphaD, const double *y,
+                             const double *hw, double search_radius,
+                             double numin, double dnu, int64_t Nx, double kmin,
+                             int64_t npts, const int64_t *jidx, double *K,
+                             int nthreads);
+
 /* ---- profiles 1-4 with optional plinth removal (all grid points) ---- */
 double hko_profile_value(int profile, double S, double x, double y, double vb);
 double hko_chi(const double *chi, double d);

@@ -42,6 +42,9 @@ def lib():
         L.hko_window.argtypes = [c_d, c_d, c_d, c_d, c_i64, _I64, _I64]
         L.hko_voigt_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d,
                                                             c_i64, _I64, _D, c_int])
+        L.hko_voigt_compat_points.argtypes = L.hko_voigt_points.argtypes
+        L.hko_compat_profile.restype = ctypes.c_float
+        L.hko_compat_profile.argtypes = [ctypes.c_float, ctypes.c_float]
         L.hko_voigt_grad.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
                                      + [c_d, c_d, c_int, c_d, c_int, c_d, c_d, c_i64, _D,
                                         _D, _D, _D, c_int])
@@ -175,6 +178,35 @@ def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
                            float(numin), float(dnu), int(Nx), float(kmin), npts, jp,
                            _p(K), int(nthreads))
     return K
+
+
+def voigt_compat_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, nthreads=0):
+    """K at grid points jidx (all if None) in the reference's own FP32
+    arithmetic (hko_voigt_compat_points), lines in centre order."""
+    cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
+    if cols[0].size > 1 and np.any(np.diff(cols[0]) < 0):
+        o = np.argsort(cols[0], kind="stable")
+        cols = [np.ascontiguousarray(c[o]) for c in cols]
+    nl = cols[0].size
+    hw = cols[4]
+    R = float(hw[np.isfinite(hw)].max()) if nl else 0.0
+    if jidx is None:
+        npts = int(Nx); jp = None
+    else:
+        jidx = np.ascontiguousarray(jidx, dtype=np.int64); npts = jidx.size
+        jp = _p(jidx, _I64)
+    K = np.empty(npts)
+    lib().hko_voigt_compat_points(nl, *[_p(c) for c in cols], R, float(numin), float(dnu),
+                                  int(Nx), float(kmin), npts, jp, _p(K), int(nthreads))
+    return K
+
+
+def compat_profile(x, y):
+    """The reference's FP32 region formulas for a unit line (no cut), per
+    (x, y) pair: Re w(x, y)/sqrt(pi) as its kernels approximate it."""
+    x = np.asarray(x, dtype=np.float32).ravel(); y = np.asarray(y, dtype=np.float32).ravel()
+    f = lib().hko_compat_profile
+    return np.array([f(float(a), float(b)) for a, b in zip(x, y)], dtype=np.float64)
 
 
 def voigt_grad(lines, T, P, cut_mode, cut, numin, dnu, Nx, gK, nthreads=0, form=0):

@@ -161,6 +161,8 @@ def test_driver_console_overrides(tmp_path):
     assert "Error: Console arguments not valid!" in out, out
     rc, out = run_driver(str(tmp_path), "-maxlines", "0")
     assert "Error: maxlines must lie in 1 .. 2147483647" in out, out
+    rc, out = run_driver(str(tmp_path), "-prec", "3")
+    assert "Error: precision must be 0, 1 or 2" in out, out
     rc, out = run_driver(str(tmp_path), "-dnu", "0.003")
     assert "range cannot be divided evenly in bins. 3333 5" in out, out
 

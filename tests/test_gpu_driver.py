@@ -49,9 +49,11 @@ def make_species(tmp, name, cids, masses, abund, n, seed, numin, numax, s_scale=
                 Qtab=[q_table(k) for k in range(len(cids))])
 
 
-def oracle_K(species, sscale, meanMass, T, P, q, gammaF, cut, numin, dnu, nx, kmin, units=0):
+def oracle_K(species, sscale, meanMass, T, P, q, gammaF, cut, numin, dnu, nx, kmin, units=0,
+             compat=False):
     """Sum over species of the oracle cross section, with readFile's
-    per-isotopologue scaling (host.h:628-644) and S2_kernel semantics."""
+    per-isotopologue scaling (host.h:628-644) and S2_kernel semantics
+    (compat: in the reference's FP32 region arithmetic)."""
     K = np.full(nx, kmin)
     for sp, ss in zip(species, sscale):
         L = sp["lines"]
@@ -66,7 +68,8 @@ def oracle_K(species, sscale, meanMass, T, P, q, gammaF, cut, numin, dnu, nx, km
         d = {k: (v[order] if isinstance(v, np.ndarray) and v.shape == order.shape else v)
              for k, v in d.items()}
         st = O.line_params(d, T, P, 0, cut)
-        K += O.voigt_points(st, numin, dnu, nx, kmin=0.0)
+        K += (O.voigt_compat_points(st, numin, dnu, nx) if compat
+              else O.voigt_points(st, numin, dnu, nx, kmin=0.0))
     return K
 
 
@@ -177,6 +180,28 @@ def test_driver_fp32c_precision_key(tmp_path):
     e = rel_err(Kd, Ko)
     print("driver FP32C K vs oracle %.3e" % e)
     assert 0.0 < e <= 1e-5
+
+
+def test_driver_compat_precision_key(tmp_path):
+    """precision = 2: the driver sums in the reference's own FP32 region
+    arithmetic (HK_VOIGT_COMPAT); its K follows the oracle's restatement of
+    it, and differs from the FP64 oracle by the reference's region-B level."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 2500, 5, 995.0, 1015.0)
+    T, P, cut = 1800.0, 2.0, 5.0
+    F.write_param_dat(str(tmp_path / "param.dat"), name="f", T=T, P=P, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=cut, doStoreFullK=1, nbins=NBINS,
+                      precision=2)
+    out = run_driver(str(tmp_path))
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    args = ([sp], [1.0], meanMass, T, P, 1.0, 1.0, cut, NUMIN, DNU, nx, 0.0)
+    Kc = oracle_K(*args, compat=True)
+    K64 = oracle_K(*args)
+    Kd = F.read_out_dat(str(tmp_path / "Out_f.dat"))[0][:, 1]
+    e, e64 = rel_err(Kd, Kc), rel_err(Kd, K64)
+    print("driver compat K vs oracle compat %.3e, vs FP64 oracle %.3e" % (e, e64))
+    assert e <= 2e-6
+    assert 1e-4 <= e64 <= 1.5e-2
 
 
 def _exomol_species(tmp, name, database, nfiles_limits, n, seed, T_table, ab=1.0, mass=18.010565,

@@ -336,3 +336,58 @@ def test_work_count_adds_up(E):
     assert w[3] > 0 and w[4] > 0 and sum(w[:4]) == tot
     for kw in (dict(mode=E.HK_VOIGT_FP32C, exact_pairs=True), dict(mode=E.HK_VOIGT_FP64)):
         assert E.work_count(st, grid, **kw) == [A, B, C, 0, 0]
+
+
+@pytest.mark.parametrize("cm, cut", [(0, 25.0), (1, 400.0)])
+def test_compat_vs_oracle(E, cm, cut):
+    """HK_VOIGT_COMPAT against the oracle's restatement of the reference's
+    FP32 arithmetic (same float operations; region C differs by the device's
+    expf/sinf/cosf last bits), and against the FP64 form by the reference's
+    own region-B error (up to 1.4e-2)."""
+    d = random_lines(20000, seed=47, numin=900.0, numax=1100.0)
+    st = E.line_params(species_from(d), 1500.0, 1.0, Q=d["iso_Q"], cut_mode=cm, cut=cut)
+    grid = E.make_grid(950.0, 1050.0, 0.002)
+    Kc = E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT).cpu().numpy()
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    Ko = O.voigt_compat_points(state_numpy(st), grid.numin, grid.dnu, grid.nx)
+    e, e64 = rel_err(Kc, Ko), rel_err(Kc, K64)
+    print("cut mode %d: compat vs oracle %.3e (%.1f %% identical), vs FP64 %.3e"
+          % (cm, e, 100.0 * np.mean(Kc == Ko), e64))
+    assert e <= 2e-6
+    assert 1e-4 <= e64 <= 1.5e-2
+    assert np.array_equal(Kc, E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT).cpu().numpy())
+
+
+def test_compat_without_core_is_bitwise(E):
+    """At 100 atm every pair has x^2 + y^2 > 100 (regions A and B only): the
+    device rounds each pair exactly as the oracle's float restatement and adds
+    in the same order, so the sums agree bit for bit."""
+    d = random_lines(5000, seed=48, numin=900.0, numax=1100.0)
+    st = E.line_params(species_from(d), 1000.0, 100.0, Q=d["iso_Q"], cut_mode=0, cut=25.0)
+    y = st.array("y").cpu().numpy()
+    assert y.min() > 10.0
+    grid = E.make_grid(950.0, 1050.0, 0.005)
+    Kc = E.voigt(st, grid, 1e-30, E.HK_VOIGT_COMPAT).cpu().numpy()
+    Ko = O.voigt_compat_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, kmin=1e-30)
+    print("identical points: %d of %d, max rel %.3e" % (np.sum(Kc == Ko), Kc.size,
+                                                       rel_err(Kc, Ko)))
+    np.testing.assert_array_equal(Kc, Ko)
+
+
+def test_compat_float_index_beyond_2_24(E):
+    """On a grid of more than 2^24 points the reference converts the point
+    index to float, so positions beyond 16777216 round to even indices; the
+    compat form does the same, matching the oracle there."""
+    d = random_lines(400, seed=49, numin=16850.0, numax=16950.0)
+    st = E.line_params(species_from(d), 1500.0, 1.0, Q=d["iso_Q"], cut_mode=0, cut=5.0)
+    grid = E.make_grid(0.0, 17000.0, 0.001)
+    assert grid.nx > 1 << 24
+    Kc = E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT).cpu().numpy()
+    j = np.arange(16840000, 16960000, 7, dtype=np.int64)
+    Ko = O.voigt_compat_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=j)
+    assert rel_err(Kc[j], Ko) <= 2e-6
+    # an odd index rounds (half to even) onto one of its even neighbours, so
+    # every line sees the same position there: about half of the (even, odd)
+    # pairs of points carry identical sums
+    jj = np.arange(16900000, 16900200, 2)
+    assert 0.4 < np.mean(Kc[jj] == Kc[jj + 1]) < 0.6

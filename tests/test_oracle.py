@@ -586,3 +586,31 @@ def test_sublorentzian_profile_factor():
     r1, r2 = Kc[j1] / K[j1], Kc[j2] / K[j2]
     assert abs((r1 / r2) / (x[j1] / x[j2]) - 1) <= 1e-13
     assert r2 < 1.0                               # wings are sub-Lorentzian
+
+
+def test_compat_region_formulas_against_exact():
+    """The restatement of the reference's FP32 region formulas
+    (Line6fA/B/C_kernel, voigt.h:1177-1706) against the exact profile
+    Re w/sqrt(pi): region B's rational approximation is off by up to 1.39e-2
+    (near x = 10, y -> 0), region A's one-term wing by ~1.6e-6 and region C's
+    FP32 Algorithm 916 by < 1e-6 -- the reference's own error levels, so a
+    dropped or mistyped term in any region fails here; x = 0 gives
+    erfcx(y)/sqrt(pi)."""
+    xs = np.concatenate([np.linspace(0.0, 12.0, 97), np.geomspace(12.0, 1000.0, 120)])
+    ys = np.concatenate([[1e-4, 1e-2], np.geomspace(0.05, 99.9, 40)])
+    worst = {"A": 0.0, "B": 0.0, "C": 0.0}
+    for y in ys:
+        c = O.compat_profile(xs, np.full(xs.size, y))
+        for x, v in zip(xs, c):
+            xf, yf = np.float32(x), np.float32(y)
+            r2 = float(xf * xf + yf * yf)
+            e = O.faddeeva(float(xf), float(yf))[0].real / np.sqrt(np.pi)
+            k = "A" if r2 >= 1e6 else ("B" if r2 >= 100.0 else "C")
+            worst[k] = max(worst[k], abs(v - e) / e)
+    print("compat worst relative error by region:", worst)
+    assert 1.35e-2 <= worst["B"] <= 1.40e-2
+    assert 1e-6 <= worst["A"] <= 2e-6
+    assert worst["C"] <= 1e-6
+    for y in (0.3, 2.0, 9.0):
+        assert O.compat_profile([0.0], [y])[0] == pytest.approx(O.erfcx(y) / np.sqrt(np.pi),
+                                                                rel=1e-6)
