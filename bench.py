@@ -254,7 +254,8 @@ def main():
 def other_paths(E, torch, arr, sp, st, grid, K, kt, w, T, evals):
     """The paths around the Voigt sum on the same workload, timed with
     events (3 runs each): device line sort of the raw list, Planck/Rosseland
-    sums, transmission of the sorted bins, and the Lorentz profile gather."""
+    sums, transmission of the sorted bins, the Lorentz profile gather, and
+    the Voigt sum in the reference's own FP32 arithmetic (compat form)."""
     out = {}
 
     def timed(fn, n=3):
@@ -288,6 +289,11 @@ def other_paths(E, torch, arr, sp, st, grid, K, kt, w, T, evals):
     ms = timed(lambda: E.profile(st, grid, 2, K=Kl), n=1)
     out["lorentz_profile"] = {"ms": ms, "pairs_per_s": evals / (ms * 1e-3),
                               "note": "FP64 gather, same cut windows as the Voigt sum"}
+    Kc = torch.empty_like(K)
+    ms = timed(lambda: E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT, K=Kc), n=1)
+    out["compat_voigt"] = {"ms": ms, "pairs_per_s": evals / (ms * 1e-3),
+                           "note": "the reference's FP32 region formulas pair by pair, FP64 "
+                                   "sums (HK_VOIGT_COMPAT)"}
     log("bench: other paths %s" % json.dumps({k: round(v["ms"], 3) for k, v in out.items()}))
     return out
 
@@ -454,16 +460,35 @@ def cpu_baseline(w, arr, T, P, grid, nlines):
     ost = O.line_params(ol, T, P, 0, w.cut)
     K = O.voigt_points(ost, grid.numin, grid.dnu, grid.nx, nthreads=ncores)
     dt = time.perf_counter() - t0
+    # the same sample in the reference's FP32-evaluation / FP64-accumulation
+    # arithmetic (the oracle's compat restatement)
+    t1 = time.perf_counter()
+    O.voigt_compat_points(ost, grid.numin, grid.dnu, grid.nx, nthreads=ncores)
+    dtc = time.perf_counter() - t1
     nu = ost["nu"]
     # evaluations of the sample: window points of each line
     j = (nu[:, None] + np.array([-w.cut, w.cut])[None, :] - grid.numin) / grid.dnu
     lo = np.clip(np.floor(j[:, 0]) + 1, 0, grid.nx); hi = np.clip(np.floor(j[:, 1]) + 1, 0, grid.nx)
     ev = float(np.sum(np.maximum(hi - lo, 0)))
-    log("bench: CPU oracle %d lines, %.3e evals in %.1f s on %d threads" % (nlines, ev, dt, ncores))
+    log("bench: CPU oracle %d lines, %.3e evals in %.1f s (compat arithmetic %.1f s) on %d "
+        "threads" % (nlines, ev, dt, dtc, ncores))
     return {"value": ev / dt, "unit": "Voigt evaluations/s", "cores": ncores, "kind": "port",
             "sample": "FP64 oracle (Algorithm 916 + continued fraction, compensated sum) over the "
                       "full %d-point grid for the first %d lines of the workload" % (grid.nx, nlines),
-            "seconds": dt, "checksum": float(np.sum(K))}
+            "seconds": dt, "checksum": float(np.sum(K)),
+            "compat_value": ev / dtc, "compat_seconds": dtc,
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
