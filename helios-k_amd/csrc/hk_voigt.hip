@@ -448,35 +448,55 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 
 // ---------------------------------------------------------------------------
 // Far field of the FP32 form.  A line whose cut window covers the whole tile
-// and whose centre lies at least kFarRatio tile half-widths from the tile's
-// centre contributes a function that is analytic over the tile with its
-// nearest singularity (the line centre) far outside it: its Chebyshev
-// interpolant on kFN nodes is exact to ~4e-9 relative (Bernstein-ellipse
-// bound rho^-10, rho = 4 + sqrt(15)).  Such lines are not evaluated point by
-// point; each adds its FP64 values at the kFN nodes, and the sum over them is
-// interpolated to the tile's points once.  The node values use the
-// asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n z^-2n, c_n = (2n-1)!!/2^n,
-// n <= 3, which requires x^2 + y^2 >= kFarR2 over the tile (first omitted
-// term of the real part (105/16) 9 y / x^10 against y / x^2: < 3e-9
-// relative).
+// and whose centre lies well away from the tile contributes a function that
+// is analytic over the tile with its nearest singularity (the line centre)
+// outside it.  Such lines are not evaluated point by point: each adds its
+// FP64 values at the tile's Chebyshev-Gauss nodes, and the sum over them is
+// interpolated to the tile's points once.  Two bands, by the distance d of
+// the centre from the tile's centre in tile half widths:
+//   band 1, d >= 4:     11 nodes, error <= 4 rho^-10 ~ 4.2e-9 relative
+//                       (Bernstein ellipse rho = 4 + sqrt(15));
+//   band 2, 2 <= d < 4: 17 nodes, error <= 4 rho^-16 ~ 3.0e-9 relative
+//                       (rho = 2 + sqrt(3)).
+// The node values use the asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n
+// z^-2n, c_n = (2n-1)!!/2^n, n <= 3, which requires x^2 + y^2 >= kFarR2 over
+// the tile (first omitted term of the real part (105/16) 9 y / x^10 against
+// y / x^2: < 3e-9 relative).
 // ---------------------------------------------------------------------------
-constexpr int kFN = 11;
-constexpr double kFarRatio = 4.0;
 constexpr double kFarR2 = 400.0;
 constexpr int kFarMinPts = 256;       // smaller (edge) tiles sum every pair
-// Chebyshev-Gauss nodes cos(pi (k + 1/2) / kFN)
-__constant__ double kFarU[kFN] = {
-    0.98982144188093273238, 0.90963199535451837141, 0.75574957435425828377,
-    0.54064081745559758211, 0.28173255684142969771, 0.0, -0.28173255684142969771,
-    -0.54064081745559758211, -0.75574957435425828377, -0.90963199535451837141,
-    -0.98982144188093273238};
+constexpr int kFN1 = 11, kFN2 = 17;
+constexpr double kFarRatio1 = 4.0, kFarRatio2 = 2.0;
+// Chebyshev-Gauss nodes cos(pi (k + 1/2) / n)
+__constant__ double kFarU1[kFN1] = {
+    0.98982144188093268422, 0.90963199535451844113, 0.7557495743542582689,
+    0.54064081745559766645, 0.28173255684142978206, 0,
+    -0.28173255684142967104, -0.54064081745559722236, -0.75574957435425815788,
+    -0.90963199535451821909, -0.98982144188093268422};
+__constant__ double kFarU2[kFN2] = {
+    0.99573417629503446769, 0.96182564317281904476, 0.89516329135506234049,
+    0.79801722728023960496, 0.67369564364655720734, 0.52643216287735605796,
+    0.36124166618715286559, 0.18374951781657053451, 0,
+    -0.18374951781657017369, -0.3612416661871529211, -0.52643216287735550285,
+    -0.67369564364655720734, -0.79801722728023949394, -0.89516329135506222947,
+    -0.96182564317281904476, -0.99573417629503457871};
+
+template <int N>
+__device__ __forceinline__ double far_u(int k) { return N == kFN1 ? kFarU1[k] : kFarU2[k]; }
+
+// One band's node positions, per-wave node sums and coefficients.
+template <int N>
+struct FarBand {
+    double xn[N];                     // node positions [cm^-1]
+    double acc[N][kThreads / 64];     // node sums, one column per wave
+    double c[N];                      // Chebyshev coefficients
+};
 
 struct FarLDS {
-    double xn[kFN];                   // node positions [cm^-1]
-    double acc[kFN][kThreads / 64];   // node sums, one column per wave
-    double c[kFN];                    // Chebyshev coefficients
+    FarBand<kFN1> b1;
+    FarBand<kFN2> b2;
     int cnt[2][kThreads / 64];        // compaction counts, alternate stages
-    int any;
+    int any1, any2;                   // band used (set to 1 by any wave)
 };
 
 // S Re w(x, y) / sqrt(pi) with Bp = S y / pi, from the asymptotic series.
@@ -524,24 +544,90 @@ __device__ __forceinline__ double wave_total64(double v) {
     return v;
 }
 
-// Whether a line (centre nu, cut half width hw, 1/alpha_D ia, y) goes
-// through the far field of the tile [j0, j0 + Wv) with half width hW
-// (points): its window covers the tile (the cut predicate |x_j - nu| < hw
-// holds at both end points, so at all of them), its centre is at least
-// kFarRatio half widths from the tile's centre and x^2 + y^2 >= kFarR2 at
-// the nearest point.  No division, so it is cheap enough to run for every
-// staged line before its window is computed.
-__device__ __forceinline__ bool far_line(double nu, double hw, double ia, double y, int64_t j0,
-                                         int64_t Wv, double hW, double numin, double dnu) {
-    if (Wv < kFarMinPts || !(hw > 0.0)) return false;
+// Far-field band (0: evaluated point by point) of a line (centre nu, cut
+// half width hw, 1/alpha_D ia, y) for the tile [j0, j0 + Wv) with half
+// width hW (points): its window covers the tile (the cut predicate
+// |x_j - nu| < hw holds at both end points, so at all of them), its centre
+// is at least kFarRatio2 half widths from the tile's centre (band 1 from
+// kFarRatio1 on) and x^2 + y^2 >= kFarR2 at the nearest point.  No division,
+// so it is cheap enough to run for every staged line before its window is
+// computed.
+__device__ __forceinline__ int far_band(double nu, double hw, double ia, double y, int64_t j0,
+                                        int64_t Wv, double hW, double numin, double dnu) {
+    if (Wv < kFarMinPts || !(hw > 0.0)) return 0;
     const double dlo = fma((double)j0, dnu, numin) - nu;
     const double dhi = fma((double)(j0 + Wv - 1), dnu, numin) - nu;
-    if (!(dlo > -hw && dlo < hw && dhi > -hw && dhi < hw)) return false;
+    if (!(dlo > -hw && dlo < hw && dhi > -hw && dhi < hw)) return 0;
     const double half = hW * dnu;
     const double d = fabs(nu - fma((double)j0 + hW, dnu, numin));
-    if (d < kFarRatio * half) return false;
+    if (d < kFarRatio2 * half) return 0;
     const double xnear = (d - half) * ia;
-    return fma(xnear, xnear, y * y) >= kFarR2;
+    if (!(fma(xnear, xnear, y * y) >= kFarR2)) return 0;
+    return d >= kFarRatio1 * half ? 1 : 2;
+}
+
+template <int N>
+__device__ __forceinline__ void band_init(FarBand<N> *b, int t, int64_t j0, double hW,
+                                          double numin, double dnu) {
+    if (t < N) b->xn[t] = fma((double)j0 + hW * (1.0 + far_u<N>(t)), dnu, numin);
+    if (t < N * (kThreads / 64)) b->acc[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0;
+}
+
+// The wave's lines of this band (mine) add their node values to the wave's
+// column of node sums (all lanes take part in the wave sums).
+template <int N>
+__device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, double fnu, double fia,
+                                           double fy, double fB, int lane, int wave) {
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        const double v = mine ? fB * far_node64((b->xn[k] - fnu) * fia, fy) : 0.0;
+        const double sw = wave_total64(v);
+        if (lane == 63) b->acc[k][wave] += sw;
+    }
+}
+
+// Node sums of the waves -> Chebyshev coefficients, by threads t0 .. t0 + N - 1
+// (two barriers are taken by the caller between the phases).
+template <int N>
+__device__ __forceinline__ void band_sum(FarBand<N> *b, int t) {
+    if (t >= 0 && t < N) {
+        double F = 0.0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) F += b->acc[t][w];
+        b->acc[t][0] = F;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void band_coeffs(FarBand<N> *b, int t) {
+    if (t >= 0 && t < N) {
+        // c_t = (2 - [t == 0]) / N sum_k F_k T_t(u_k), T_t by its recurrence
+        double c = 0.0;
+        for (int k = 0; k < N; ++k) {
+            const double u = far_u<N>(k);
+            double tm = 1.0, tc = u;
+            for (int m = 2; m <= t; ++m) {
+                const double tn = fma(2.0 * u, tc, -tm);
+                tm = tc;
+                tc = tn;
+            }
+            c += b->acc[k][0] * (t == 0 ? 1.0 : tc);
+        }
+        b->c[t] = (t == 0 ? 1.0 : 2.0) / (double)N * c;
+    }
+}
+
+// Clenshaw sum of the band's interpolant at u in [-1, 1].
+template <int N>
+__device__ __forceinline__ double band_eval(const FarBand<N> *b, double u) {
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int m = N - 1; m >= 1; --m) {
+        const double b0 = fma(2.0 * u, b1, b->c[m] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(u, b1, b->c[0] - b2);
 }
 
 template <int PPT>
@@ -562,9 +648,9 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     const int64_t Wv = min((int64_t)W, a.nx - j0);
     const double hW = 0.5 * (double)(Wv - 1);
     const bool ffok = a.far && Wv >= kFarMinPts;
-    if (t < kFN) fl->xn[t] = fma((double)j0 + hW * (1.0 + kFarU[t]), a.dnu, a.numin);
-    if (t < kFN * (kThreads / 64)) fl->acc[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0;
-    if (t == 0) fl->any = 0;
+    band_init<kFN1>(&fl->b1, t, j0, hW, a.numin, a.dnu);
+    band_init<kFN2>(&fl->b2, t, j0, hW, a.numin, a.dnu);
+    if (t == 0) { fl->any1 = 0; fl->any2 = 0; }
     __syncthreads();
     int nprev = 0, par = 0;
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
@@ -576,21 +662,22 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         // registers are free again before the direct records are built)
         if (ffok) {
             double fy = 0.0, fnu = 0.0, fia = 0.0, fB = 0.0;
+            int band = 0;
             if (i < ti.hi) {
                 fnu = a.nu[i];
                 fia = a.ia[i];
                 fy = a.y[i];
-                far = far_line(fnu, a.hw[i], fia, fy, j0, Wv, hW, a.numin, a.dnu);
-                if (far) fB = ldexp(a.S[i], -ti.e);
+                band = far_band(fnu, a.hw[i], fia, fy, j0, Wv, hW, a.numin, a.dnu);
+                if (band) fB = ldexp(a.S[i], -ti.e);
             }
-            if (__ballot(far) != 0ull) {
-#pragma unroll 1
-                for (int k = 0; k < kFN; ++k) {
-                    const double v = far ? fB * far_node64((fl->xn[k] - fnu) * fia, fy) : 0.0;
-                    const double sw = wave_total64(v);
-                    if (lane == 63) fl->acc[k][wave] += sw;
-                }
-                if (lane == 0) fl->any = 1;
+            far = band != 0;
+            if (__ballot(band == 1) != 0ull) {
+                band_nodes<kFN1>(&fl->b1, band == 1, fnu, fia, fy, fB, lane, wave);
+                if (lane == 0) fl->any1 = 1;
+            }
+            if (__ballot(band == 2) != 0ull) {
+                band_nodes<kFN2>(&fl->b2, band == 2, fnu, fia, fy, fB, lane, wave);
+                if (lane == 0) fl->any2 = 1;
             }
         }
         if (i < ti.hi && !far) {
@@ -675,30 +762,14 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     }
     // far field: node sums -> Chebyshev coefficients -> the tile's points
     __syncthreads();
-    const bool any = fl->any != 0;
+    const int any = fl->any1 | (fl->any2 << 1);
     if (any) {
-        if (t < kFN) {
-            double F = 0.0;
-#pragma unroll
-            for (int w = 0; w < kThreads / 64; ++w) F += fl->acc[t][w];
-            fl->acc[t][0] = F;
-        }
+        // band 1 on threads 0..10, band 2 on threads 64..80 (another wave)
+        band_sum<kFN1>(&fl->b1, t);
+        band_sum<kFN2>(&fl->b2, t - 64);
         __syncthreads();
-        if (t < kFN) {
-            // c_t = (2 - [t == 0]) / kFN sum_k F_k T_t(u_k), T_t by its recurrence
-            double c = 0.0;
-            for (int k = 0; k < kFN; ++k) {
-                const double u = kFarU[k];
-                double tm = 1.0, tc = u;
-                for (int m = 2; m <= t; ++m) {
-                    const double tn = fma(2.0 * u, tc, -tm);
-                    tm = tc;
-                    tc = tn;
-                }
-                c += fl->acc[k][0] * (t == 0 ? 1.0 : tc);
-            }
-            fl->c[t] = (t == 0 ? 1.0 : 2.0) / (double)kFN * c;
-        }
+        band_coeffs<kFN1>(&fl->b1, t);
+        band_coeffs<kFN2>(&fl->b2, t - 64);
         __syncthreads();
     }
 #pragma unroll
@@ -707,16 +778,10 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         const int64_t j = j0 + jr0 + 64 * k;
         double ff = 0.0;
         if (any) {
-            // Clenshaw at u = (jr - hW) / hW
+            // the interpolants at u = (jr - hW) / hW
             const double u = ((double)(jr0 + 64 * k) - hW) / hW;
-            double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-            for (int m = kFN - 1; m >= 1; --m) {
-                const double b0 = fma(2.0 * u, b1, fl->c[m] - b2);
-                b2 = b1;
-                b1 = b0;
-            }
-            ff = fma(u, b1, fl->c[0] - b2);
+            if (any & 1) ff += band_eval<kFN1>(&fl->b1, u);
+            if (any & 2) ff += band_eval<kFN2>(&fl->b2, u);
         }
         if (j < a.nx) {
             const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
@@ -1004,8 +1069,10 @@ __global__ void zero4_kernel(unsigned long long *c) {
 
 // Work of one FP32 Voigt sum with the far field at tile width W: per line,
 // the tiles its window touches, split into far tiles (pairs summed through
-// the far field, node evaluations) and direct pairs by region.  out: direct
-// A, B, C pairs, far-field pairs, far (line, tile) count.
+// the far field, node evaluations) and direct pairs by region.  The band
+// decision is the kernel's (far_band on the same FP64 values).  out: direct
+// A, B, C pairs, far-field pairs, far (line, tile) counts of band 1 (11
+// nodes) and band 2 (17 nodes).
 __global__ __launch_bounds__(256) void work_count_kernel(const double *__restrict__ nu,
                                                          const double *__restrict__ hw,
                                                          const double *__restrict__ ia,
@@ -1013,10 +1080,10 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                                                          double numin, double dnu, int64_t nx,
                                                          int W, int far,
                                                          unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long red[5][256];
+    __shared__ unsigned long long red[6][256];
     const int t = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * 256 + t;
-    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0;
+    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0, fl2 = 0;
     if (i < nl) {
         int64_t wl, wh, bl = 0, bh = 0, cl = 0, ch = 0;
         cut_window(nu[i], hw[i], numin, dnu, nx, wl, wh);
@@ -1031,9 +1098,10 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
             for (int64_t tl = wl / W; tl <= (wh - 1) / W; ++tl) {
                 const int64_t j0 = tl * W, Wv = min((int64_t)W, nx - j0);
                 const double hW = 0.5 * (double)(Wv - 1);
-                if (far && far_line(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu)) {
+                const int band = far ? far_band(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu) : 0;
+                if (band) {
                     fp += (unsigned long long)Wv;
-                    fl += 1;
+                    if (band == 1) fl += 1; else fl2 += 1;
                     continue;
                 }
                 const unsigned long long tot = ov(wl, wh, j0, j0 + Wv);
@@ -1046,13 +1114,14 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
         }
     }
     red[0][t] = cA; red[1][t] = cB; red[2][t] = cC; red[3][t] = fp; red[4][t] = fl;
+    red[5][t] = fl2;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
-            for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + s];
+            for (int q = 0; q < 6; ++q) red[q][t] += red[q][t + s];
         __syncthreads();
     }
-    if (t < 5) atomicAdd(&out[t], red[t][0]);
+    if (t < 6) atomicAdd(&out[t], red[t][0]);
 }
 
 __global__ void zero8_kernel(unsigned long long *c) {
