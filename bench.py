@@ -36,10 +36,11 @@ for p in (ROOT, os.path.join(ROOT, "helios-k_amd")):
 FLOPS_PER_EVAL = {"A": 10.0, "B": 22.0, "C": 400.0}
 # Far field of the FP32C form: one FP64 node evaluation (4-term complex
 # asymptotic series with a refined reciprocal, ~40 FP64 operations, counted
-# twice at the FP32 rate) per node, 11 nodes per (line, tile) in band 1 and
-# 17 in band 2; an 11-term Clenshaw sum per point.
+# twice at the FP32 rate) per node, 11 nodes per (line, tile) in band 1, 17
+# in band 2 and 23 in band 3; an 11-term Clenshaw sum per point.
 FAR_NODES = 11
 FAR_NODES2 = 17
+FAR_NODES3 = 23
 FLOPS_PER_NODE = 80.0
 FLOPS_PER_POINT_FAR = 25.0
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
@@ -141,11 +142,12 @@ def main():
     work = E.work_count(st, grid, E.HK_VOIGT_FP32C)
     direct = {"A": work[0], "B": work[1], "C": work[2]}
     flops = (sum(direct[r] * FLOPS_PER_EVAL[r] for r in direct)
-             + (work[4] * FAR_NODES + work[5] * FAR_NODES2) * FLOPS_PER_NODE
+             + (work[4] * FAR_NODES + work[5] * FAR_NODES2 + work[6] * FAR_NODES3)
+             * FLOPS_PER_NODE
              + grid.nx * FLOPS_PER_POINT_FAR)
     log("bench: %.4e evaluations per step, regions A/B/C = %s" % (evals, region))
     log("bench: FP32C work: direct pairs A/B/C = %s, far-field pairs %.4e, far (line, tile) "
-        "%.4e (11 nodes) + %.4e (17 nodes)" % (direct, work[3], work[4], work[5]))
+        "%.4e (11 nodes) + %.4e (17) + %.4e (23)" % (direct, work[3], work[4], work[5], work[6]))
 
     def step(mode, ev=None):
         E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut, state=st)
@@ -227,8 +229,10 @@ def main():
         "voigt_evals_per_step": evals,
         "voigt_regions_ABC": [region["A"], region["B"], region["C"]],
         "voigt_work": {"direct_pairs_ABC": [direct["A"], direct["B"], direct["C"]],
-                       "far_field_pairs": work[3], "far_line_tiles": work[4] + work[5],
-                       "far_node_evaluations": work[4] * FAR_NODES + work[5] * FAR_NODES2},
+                       "far_field_pairs": work[3],
+                       "far_line_tiles": work[4] + work[5] + work[6],
+                       "far_node_evaluations": (work[4] * FAR_NODES + work[5] * FAR_NODES2
+                                                + work[6] * FAR_NODES3)},
         "voigt_kernel_ms": voigt_ms,
         "voigt_evals_per_s_kernel": evals / (voigt_ms * 1e-3),
         "fp64_voigt_kernel_ms": fp64_ms,

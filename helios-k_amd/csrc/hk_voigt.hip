@@ -452,12 +452,14 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // is analytic over the tile with its nearest singularity (the line centre)
 // outside it.  Such lines are not evaluated point by point: each adds its
 // FP64 values at the tile's Chebyshev-Gauss nodes, and the sum over them is
-// interpolated to the tile's points once.  Two bands, by the distance d of
-// the centre from the tile's centre in tile half widths:
-//   band 1, d >= 4:     11 nodes, error <= 4 rho^-10 ~ 4.2e-9 relative
-//                       (Bernstein ellipse rho = 4 + sqrt(15));
-//   band 2, 2 <= d < 4: 17 nodes, error <= 4 rho^-16 ~ 3.0e-9 relative
-//                       (rho = 2 + sqrt(3)).
+// interpolated to the tile's points once.  Three bands, by the distance d
+// of the centre from the tile's centre in tile half widths:
+//   band 1, d >= 4:       11 nodes, error <= 4 rho^-10 ~ 4.2e-9 relative
+//                         (Bernstein ellipse rho = 4 + sqrt(15));
+//   band 2, 2 <= d < 4:   17 nodes, error <= 4 rho^-16 ~ 3.0e-9 relative
+//                         (rho = 2 + sqrt(3));
+//   band 3, 1.5 <= d < 2: 23 nodes, error <= 4 rho^-22 ~ 2.6e-9 relative
+//                         (rho = 1.5 + sqrt(1.25)).
 // The node values use the asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n
 // z^-2n, c_n = (2n-1)!!/2^n, n <= 3, which requires x^2 + y^2 >= kFarR2 over
 // the tile (first omitted term of the real part (105/16) 9 y / x^10 against
@@ -465,8 +467,8 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // ---------------------------------------------------------------------------
 constexpr double kFarR2 = 400.0;
 constexpr int kFarMinPts = 256;       // smaller (edge) tiles sum every pair
-constexpr int kFN1 = 11, kFN2 = 17;
-constexpr double kFarRatio1 = 4.0, kFarRatio2 = 2.0;
+constexpr int kFN1 = 11, kFN2 = 17, kFN3 = 23;
+constexpr double kFarRatio1 = 4.0, kFarRatio2 = 2.0, kFarRatio3 = 1.5;
 // Chebyshev-Gauss nodes cos(pi (k + 1/2) / n)
 __constant__ double kFarU1[kFN1] = {
     0.98982144188093268422, 0.90963199535451844113, 0.7557495743542582689,
@@ -481,8 +483,19 @@ __constant__ double kFarU2[kFN2] = {
     -0.67369564364655720734, -0.79801722728023949394, -0.89516329135506222947,
     -0.96182564317281904476, -0.99573417629503457871};
 
+__constant__ double kFarU3[kFN3] = {
+    0.99766876919053915085, 0.97908408768232291219, 0.94226092211882050975,
+    0.88788521840237522031, 0.81696989301044209419, 0.73083596427812413943,
+    0.63108794432605275215, 0.51958395003543356161, 0.39840108984624134081,
+    0.26979677115702443979, 0.13616664909624659474, 0,
+    -0.13616664909624667801, -0.26979677115702410672, -0.39840108984624145183,
+    -0.51958395003543333956, -0.63108794432605286318, -0.73083596427812413943,
+    -0.81696989301044198317, -0.88788521840237522031, -0.94226092211882039873,
+    -0.97908408768232280117, -0.99766876919053926187};
 template <int N>
-__device__ __forceinline__ double far_u(int k) { return N == kFN1 ? kFarU1[k] : kFarU2[k]; }
+__device__ __forceinline__ double far_u(int k) {
+    return N == kFN1 ? kFarU1[k] : (N == kFN2 ? kFarU2[k] : kFarU3[k]);
+}
 
 // One band's node positions, per-wave node sums and coefficients.
 template <int N>
@@ -495,8 +508,9 @@ struct FarBand {
 struct FarLDS {
     FarBand<kFN1> b1;
     FarBand<kFN2> b2;
+    FarBand<kFN3> b3;
     int cnt[2][kThreads / 64];        // compaction counts, alternate stages
-    int any1, any2;                   // band used (set to 1 by any wave)
+    int any1, any2, any3;             // band used (set to 1 by any wave)
 };
 
 // S Re w(x, y) / sqrt(pi) with Bp = S y / pi, from the asymptotic series.
@@ -548,8 +562,9 @@ __device__ __forceinline__ double wave_total64(double v) {
 // half width hw, 1/alpha_D ia, y) for the tile [j0, j0 + Wv) with half
 // width hW (points): its window covers the tile (the cut predicate
 // |x_j - nu| < hw holds at both end points, so at all of them), its centre
-// is at least kFarRatio2 half widths from the tile's centre (band 1 from
-// kFarRatio1 on) and x^2 + y^2 >= kFarR2 at the nearest point.  No division,
+// is at least kFarRatio3 half widths from the tile's centre (band 2 from
+// kFarRatio2 on, band 1 from kFarRatio1 on) and x^2 + y^2 >= kFarR2 at the
+// nearest point.  No division,
 // so it is cheap enough to run for every staged line before its window is
 // computed.
 __device__ __forceinline__ int far_band(double nu, double hw, double ia, double y, int64_t j0,
@@ -560,10 +575,10 @@ __device__ __forceinline__ int far_band(double nu, double hw, double ia, double 
     if (!(dlo > -hw && dlo < hw && dhi > -hw && dhi < hw)) return 0;
     const double half = hW * dnu;
     const double d = fabs(nu - fma((double)j0 + hW, dnu, numin));
-    if (d < kFarRatio2 * half) return 0;
+    if (d < kFarRatio3 * half) return 0;
     const double xnear = (d - half) * ia;
     if (!(fma(xnear, xnear, y * y) >= kFarR2)) return 0;
-    return d >= kFarRatio1 * half ? 1 : 2;
+    return d >= kFarRatio1 * half ? 1 : (d >= kFarRatio2 * half ? 2 : 3);
 }
 
 template <int N>
@@ -650,7 +665,8 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     const bool ffok = a.far && Wv >= kFarMinPts;
     band_init<kFN1>(&fl->b1, t, j0, hW, a.numin, a.dnu);
     band_init<kFN2>(&fl->b2, t, j0, hW, a.numin, a.dnu);
-    if (t == 0) { fl->any1 = 0; fl->any2 = 0; }
+    band_init<kFN3>(&fl->b3, t, j0, hW, a.numin, a.dnu);
+    if (t == 0) { fl->any1 = 0; fl->any2 = 0; fl->any3 = 0; }
     __syncthreads();
     int nprev = 0, par = 0;
     for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
@@ -678,6 +694,10 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             if (__ballot(band == 2) != 0ull) {
                 band_nodes<kFN2>(&fl->b2, band == 2, fnu, fia, fy, fB, lane, wave);
                 if (lane == 0) fl->any2 = 1;
+            }
+            if (__ballot(band == 3) != 0ull) {
+                band_nodes<kFN3>(&fl->b3, band == 3, fnu, fia, fy, fB, lane, wave);
+                if (lane == 0) fl->any3 = 1;
             }
         }
         if (i < ti.hi && !far) {
@@ -762,14 +782,16 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     }
     // far field: node sums -> Chebyshev coefficients -> the tile's points
     __syncthreads();
-    const int any = fl->any1 | (fl->any2 << 1);
+    const int any = fl->any1 | (fl->any2 << 1) | (fl->any3 << 2);
     if (any) {
-        // band 1 on threads 0..10, band 2 on threads 64..80 (another wave)
+        // band 1 on threads 0..10, band 2 on 64..80, band 3 on 128..150
         band_sum<kFN1>(&fl->b1, t);
         band_sum<kFN2>(&fl->b2, t - 64);
+        band_sum<kFN3>(&fl->b3, t - 128);
         __syncthreads();
         band_coeffs<kFN1>(&fl->b1, t);
         band_coeffs<kFN2>(&fl->b2, t - 64);
+        band_coeffs<kFN3>(&fl->b3, t - 128);
         __syncthreads();
     }
 #pragma unroll
@@ -782,6 +804,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             const double u = ((double)(jr0 + 64 * k) - hW) / hW;
             if (any & 1) ff += band_eval<kFN1>(&fl->b1, u);
             if (any & 2) ff += band_eval<kFN2>(&fl->b2, u);
+            if (any & 4) ff += band_eval<kFN3>(&fl->b3, u);
         }
         if (j < a.nx) {
             const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
@@ -1072,7 +1095,7 @@ __global__ void zero4_kernel(unsigned long long *c) {
 // the far field, node evaluations) and direct pairs by region.  The band
 // decision is the kernel's (far_band on the same FP64 values).  out: direct
 // A, B, C pairs, far-field pairs, far (line, tile) counts of band 1 (11
-// nodes) and band 2 (17 nodes).
+// nodes), band 2 (17 nodes) and band 3 (23 nodes).
 __global__ __launch_bounds__(256) void work_count_kernel(const double *__restrict__ nu,
                                                          const double *__restrict__ hw,
                                                          const double *__restrict__ ia,
@@ -1080,10 +1103,10 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                                                          double numin, double dnu, int64_t nx,
                                                          int W, int far,
                                                          unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long red[6][256];
+    __shared__ unsigned long long red[7][256];
     const int t = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * 256 + t;
-    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0, fl2 = 0;
+    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0, fl2 = 0, fl3 = 0;
     if (i < nl) {
         int64_t wl, wh, bl = 0, bh = 0, cl = 0, ch = 0;
         cut_window(nu[i], hw[i], numin, dnu, nx, wl, wh);
@@ -1101,7 +1124,7 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                 const int band = far ? far_band(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu) : 0;
                 if (band) {
                     fp += (unsigned long long)Wv;
-                    if (band == 1) fl += 1; else fl2 += 1;
+                    if (band == 1) fl += 1; else if (band == 2) fl2 += 1; else fl3 += 1;
                     continue;
                 }
                 const unsigned long long tot = ov(wl, wh, j0, j0 + Wv);
@@ -1114,14 +1137,14 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
         }
     }
     red[0][t] = cA; red[1][t] = cB; red[2][t] = cC; red[3][t] = fp; red[4][t] = fl;
-    red[5][t] = fl2;
+    red[5][t] = fl2; red[6][t] = fl3;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
-            for (int q = 0; q < 6; ++q) red[q][t] += red[q][t + s];
+            for (int q = 0; q < 7; ++q) red[q][t] += red[q][t + s];
         __syncthreads();
     }
-    if (t < 6) atomicAdd(&out[t], red[t][0]);
+    if (t < 7) atomicAdd(&out[t], red[t][0]);
 }
 
 __global__ void zero8_kernel(unsigned long long *c) {
