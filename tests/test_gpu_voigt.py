@@ -391,3 +391,19 @@ def test_compat_float_index_beyond_2_24(E):
     # pairs of points carry identical sums
     jj = np.arange(16900000, 16900200, 2)
     assert 0.4 < np.mean(Kc[jj] == Kc[jj + 1]) < 0.6
+
+
+def test_compat_cut_mode_2_accumulate_and_empty_lines(E):
+    """Compat form with cut mode 2 (cut in Doppler widths), lines at nu = 0
+    (empty: no window), kmin and accumulation onto an existing K."""
+    d = random_lines(6000, seed=50, numin=900.0, numax=1100.0)
+    d["nu"][:3] = 0.0
+    st = E.line_params(species_from(d), 900.0, 0.3, Q=d["iso_Q"], cut_mode=2, cut=300.0)
+    grid = E.make_grid(950.0, 1050.0, 0.002)
+    Ko = O.voigt_compat_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, kmin=1e-28)
+    Kc = E.voigt(st, grid, 1e-28, E.HK_VOIGT_COMPAT).cpu().numpy()
+    assert rel_err(Kc, Ko) <= 2e-6
+    import torch
+    K0 = torch.full((grid.nx,), 5e-25, dtype=torch.float64, device="cuda")
+    Ka = E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT, K=K0, accumulate=True).cpu().numpy()
+    assert rel_err(Ka - 5e-25, Ko - 1e-28) <= 2e-6
