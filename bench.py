@@ -36,11 +36,11 @@ for p in (ROOT, os.path.join(ROOT, "helios-k_amd")):
 FLOPS_PER_EVAL = {"A": 10.0, "B": 22.0, "C": 400.0}
 # Far field of the FP32C form: one FP64 node evaluation (4-term complex
 # asymptotic series with a refined reciprocal, ~40 FP64 operations, counted
-# twice at the FP32 rate) per node, 11 nodes per (line, tile) in band 1, 17
-# in band 2 and 23 in band 3; an 11-term Clenshaw sum per point.
-FAR_NODES = 11
-FAR_NODES2 = 17
-FAR_NODES3 = 23
+# twice at the FP32 rate) per node, 9 nodes per (line, tile) in band 1, 15
+# in band 2 and 19 in band 3; Clenshaw sums per point.
+FAR_NODES = 9
+FAR_NODES2 = 15
+FAR_NODES3 = 19
 FLOPS_PER_NODE = 80.0
 FLOPS_PER_POINT_FAR = 25.0
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32
@@ -147,7 +147,7 @@ def main():
              + grid.nx * FLOPS_PER_POINT_FAR)
     log("bench: %.4e evaluations per step, regions A/B/C = %s" % (evals, region))
     log("bench: FP32C work: direct pairs A/B/C = %s, far-field pairs %.4e, far (line, tile) "
-        "%.4e (11 nodes) + %.4e (17) + %.4e (23)" % (direct, work[3], work[4], work[5], work[6]))
+        "%.4e (9 nodes) + %.4e (15) + %.4e (19)" % (direct, work[3], work[4], work[5], work[6]))
 
     def step(mode, ev=None):
         E.line_params(sp, T, P, Q=[1.0], cut_mode=0, cut=w.cut, state=st)

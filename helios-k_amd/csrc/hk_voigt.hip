@@ -454,12 +454,14 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // FP64 values at the tile's Chebyshev-Gauss nodes, and the sum over them is
 // interpolated to the tile's points once.  Three bands, by the distance d
 // of the centre from the tile's centre in tile half widths:
-//   band 1, d >= 4:       11 nodes, error <= 4 rho^-10 ~ 4.2e-9 relative
+//   band 1, d >= 4:       9 nodes, error <= 4 rho^-8 ~ 2.7e-7 relative
 //                         (Bernstein ellipse rho = 4 + sqrt(15));
-//   band 2, 2 <= d < 4:   17 nodes, error <= 4 rho^-16 ~ 3.0e-9 relative
+//   band 2, 2 <= d < 4:   15 nodes, error <= 4 rho^-14 ~ 3.9e-8 relative
 //                         (rho = 2 + sqrt(3));
-//   band 3, 1.5 <= d < 2: 23 nodes, error <= 4 rho^-22 ~ 2.6e-9 relative
-//                         (rho = 1.5 + sqrt(1.25)).
+//   band 3, 1.5 <= d < 2: 19 nodes, error <= 4 rho^-18 ~ 1.2e-7 relative
+//                         (rho = 1.5 + sqrt(1.25));
+// each of a far line's own contribution, which is positive, so the far
+// field's share of K carries at most 2.7e-7.
 // The node values use the asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n
 // z^-2n, c_n = (2n-1)!!/2^n, n <= 3, which requires x^2 + y^2 >= kFarR2 over
 // the tile (first omitted term of the real part (105/16) 9 y / x^10 against
@@ -467,31 +469,28 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // ---------------------------------------------------------------------------
 constexpr double kFarR2 = 400.0;
 constexpr int kFarMinPts = 256;       // smaller (edge) tiles sum every pair
-constexpr int kFN1 = 11, kFN2 = 17, kFN3 = 23;
+constexpr int kFN1 = 9, kFN2 = 15, kFN3 = 19;
 constexpr double kFarRatio1 = 4.0, kFarRatio2 = 2.0, kFarRatio3 = 1.5;
 // Chebyshev-Gauss nodes cos(pi (k + 1/2) / n)
 __constant__ double kFarU1[kFN1] = {
-    0.98982144188093268422, 0.90963199535451844113, 0.7557495743542582689,
-    0.54064081745559766645, 0.28173255684142978206, 0,
-    -0.28173255684142967104, -0.54064081745559722236, -0.75574957435425815788,
-    -0.90963199535451821909, -0.98982144188093268422};
+    0.98480775301220802032, 0.86602540378443870761, 0.64278760968653936292,
+    0.34202014332566882393, 0, -0.34202014332566849086,
+    -0.64278760968653936292, -0.86602540378443848557, -0.98480775301220802032};
 __constant__ double kFarU2[kFN2] = {
-    0.99573417629503446769, 0.96182564317281904476, 0.89516329135506234049,
-    0.79801722728023960496, 0.67369564364655720734, 0.52643216287735605796,
-    0.36124166618715286559, 0.18374951781657053451, 0,
-    -0.18374951781657017369, -0.3612416661871529211, -0.52643216287735550285,
-    -0.67369564364655720734, -0.79801722728023949394, -0.89516329135506222947,
-    -0.96182564317281904476, -0.99573417629503457871};
+    0.99452189536827328986, 0.95105651629515353118, 0.86602540378443870761,
+    0.74314482547739424412, 0.5877852522924731371, 0.4067366430758003748,
+    0.20791169081775923155, 0, -0.20791169081775934258,
+    -0.40673664307580004174, -0.58778525229247302608, -0.74314482547739402207,
+    -0.86602540378443870761, -0.95105651629515353118, -0.99452189536827340088};
 
 __constant__ double kFarU3[kFN3] = {
-    0.99766876919053915085, 0.97908408768232291219, 0.94226092211882050975,
-    0.88788521840237522031, 0.81696989301044209419, 0.73083596427812413943,
-    0.63108794432605275215, 0.51958395003543356161, 0.39840108984624134081,
-    0.26979677115702443979, 0.13616664909624659474, 0,
-    -0.13616664909624667801, -0.26979677115702410672, -0.39840108984624145183,
-    -0.51958395003543333956, -0.63108794432605286318, -0.73083596427812413943,
-    -0.81696989301044198317, -0.88788521840237522031, -0.94226092211882039873,
-    -0.97908408768232280117, -0.99766876919053926187};
+    0.99658449300666984705, 0.96940026593933037447, 0.91577332665505739584,
+    0.83716647826252865716, 0.73572391067313169799, 0.61421271268966792789,
+    0.47594739303707345179, 0.32469946920468356621, 0.16459459028073375464,
+    0, -0.16459459028073383791, -0.32469946920468323315,
+    -0.47594739303707356282, -0.61421271268966781687, -0.73572391067313158697,
+    -0.83716647826252832409, -0.91577332665505750686, -0.96940026593933037447,
+    -0.99658449300666984705};
 template <int N>
 __device__ __forceinline__ double far_u(int k) {
     return N == kFN1 ? kFarU1[k] : (N == kFN2 ? kFarU2[k] : kFarU3[k]);
@@ -784,7 +783,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     __syncthreads();
     const int any = fl->any1 | (fl->any2 << 1) | (fl->any3 << 2);
     if (any) {
-        // band 1 on threads 0..10, band 2 on 64..80, band 3 on 128..150
+        // band 1 on threads 0..8, band 2 on 64..78, band 3 on 128..146
         band_sum<kFN1>(&fl->b1, t);
         band_sum<kFN2>(&fl->b2, t - 64);
         band_sum<kFN3>(&fl->b3, t - 128);
@@ -1094,8 +1093,8 @@ __global__ void zero4_kernel(unsigned long long *c) {
 // the tiles its window touches, split into far tiles (pairs summed through
 // the far field, node evaluations) and direct pairs by region.  The band
 // decision is the kernel's (far_band on the same FP64 values).  out: direct
-// A, B, C pairs, far-field pairs, far (line, tile) counts of band 1 (11
-// nodes), band 2 (17 nodes) and band 3 (23 nodes).
+// A, B, C pairs, far-field pairs, far (line, tile) counts of band 1 (9
+// nodes), band 2 (15 nodes) and band 3 (19 nodes).
 __global__ __launch_bounds__(256) void work_count_kernel(const double *__restrict__ nu,
                                                          const double *__restrict__ hw,
                                                          const double *__restrict__ ia,

@@ -200,8 +200,8 @@ int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
 /* Work actually done by one hk_voigt_accumulate call with this mode and
  * tuning (tile width set by tile_points): counts (8 device uint64) =
  * [direct pairs in regions A, B, C (as hk_voigt_pair_count), pairs summed
- * through the FP32 far field, far (line, tile) pairs of band 1 -- each 11
- * node evaluations --, of band 2 -- 17 each --, of band 3 -- 23 each --,
+ * through the FP32 far field, far (line, tile) pairs of band 1 -- each 9
+ * node evaluations --, of band 2 -- 15 each --, of band 3 -- 19 each --,
  * 0].  The direct and far-field pairs add up to counts[0] of
  * hk_voigt_pair_count. */
 int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
