@@ -93,8 +93,8 @@ def main():
                          "the default is about 10 s on 16 threads at the default workload)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the path shards by wavenumber range or (T,P) with no exchange; see DESIGN.md)"
+        print("bench.py: measures one GPU; --gpus %d is not supported (the path would shard "
+              "by wavenumber range or by (T,P) spectrum, with no exchange between GPUs)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 

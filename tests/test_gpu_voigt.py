@@ -12,7 +12,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-11      # plain FP64 form vs the oracle
-FP32C_TOL = 1e-5      # FP32 compensated form vs the oracle
+FP32C_TOL = 1e-5      # FP32 compensated form vs the oracle (the required bound)
 
 
 @pytest.fixture(scope="module")
