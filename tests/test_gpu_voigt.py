@@ -407,3 +407,27 @@ def test_compat_cut_mode_2_accumulate_and_empty_lines(E):
     K0 = torch.full((grid.nx,), 5e-25, dtype=torch.float64, device="cuda")
     Ka = E.voigt(st, grid, 0.0, E.HK_VOIGT_COMPAT, K=K0, accumulate=True).cpu().numpy()
     assert rel_err(Ka - 5e-25, Ko - 1e-28) <= 2e-6
+
+
+@pytest.mark.parametrize("k", [0, 15])
+def test_mixed_width_lines_vs_oracle(E, k):
+    """The heavy-tailed generator of the mixed-width workload (widths
+    log-uniform over four decades, 5 % strong lines) at its coldest, lowest
+    pressure and its hottest, highest pressure points, 2e5 lines on the full
+    4e6-point grid: FP64 and FP32C against the oracle on 3000 random points,
+    FP32C against FP64 everywhere."""
+    from heliosk import configs
+    w = configs.WORKLOADS["c4"]
+    arr = configs.species_arrays(w, 0, n=200_000)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    T, Pbar = w.tp[k]
+    st = E.line_params(sp, T, Pbar * configs.POBAR, Q=[1.0], cut_mode=0, cut=w.cut)
+    grid = _grid(E, w.numin, w.numax, w.dnu)
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    K32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, tile_points=512).cpu().numpy()
+    jj = np.sort(np.random.default_rng(2).choice(grid.nx, 3000, replace=False))
+    Ko = O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj)
+    e64, e32, e3264 = rel_err(K64[jj], Ko), rel_err(K32[jj], Ko), rel_err(K32, K64)
+    print("mixed widths (T %.0f K, P %.3g bar): FP64 vs oracle %.3e, FP32C vs oracle %.3e, "
+          "FP32C vs FP64 %.3e" % (T, Pbar, e64, e32, e3264))
+    assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
