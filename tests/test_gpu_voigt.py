@@ -431,3 +431,32 @@ def test_mixed_width_lines_vs_oracle(E, k):
     print("mixed widths (T %.0f K, P %.3g bar): FP64 vs oracle %.3e, FP32C vs oracle %.3e, "
           "FP32C vs FP64 %.3e" % (T, Pbar, e64, e32, e3264))
     assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
+
+
+def test_batched_extreme_tp_vs_oracle(E):
+    """The batched form at the batched workload's extreme points (100 K,
+    1e-6 bar and 3000 K, 1e3 bar) for one of its species (1e5 lines, 16 amu)
+    on the full 3e6-point grid: both forms against the oracle on 3000 random
+    points, FP32C against FP64 everywhere."""
+    import torch
+    from heliosk import configs
+    w = configs.WORKLOADS["c2"]
+    arr = configs.species_arrays(w, 1)
+    sp = E.Species(arr["nu"], arr["S"], arr["EL"], arr["vy"], arr["n"], iso_mass=arr["iso_mass"])
+    grid = _grid(E, w.numin, w.numax, w.dnu)
+    sts = [E.line_params(sp, T, Pbar * configs.POBAR, Q=[1.0], cut_mode=0, cut=w.cut)
+           for T, Pbar in (w.tp[0], w.tp[-1])]
+    jj = np.sort(np.random.default_rng(3).choice(grid.nx, 3000, replace=False))
+    Kos = [O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj) for st in sts]
+    res = {}
+    for mode in (E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C):
+        outs = [torch.empty(grid.nx, dtype=torch.float64, device="cuda") for _ in sts]
+        E.voigt_batched([(st, grid, 0.0, K) for st, K in zip(sts, outs)], mode=mode)
+        res[mode] = [K.cpu().numpy() for K in outs]
+        for K, Ko, tp in zip(res[mode], Kos, (w.tp[0], w.tp[-1])):
+            e = rel_err(K[jj], Ko)
+            print("batched %s at T %.0f K, P %.3g bar: vs oracle %.3e"
+                  % ("FP64" if mode == E.HK_VOIGT_FP64 else "FP32C", tp[0], tp[1], e))
+            assert e <= (FP64_TOL if mode == E.HK_VOIGT_FP64 else FP32C_TOL)
+    for K32, K64 in zip(res[E.HK_VOIGT_FP32C], res[E.HK_VOIGT_FP64]):
+        assert rel_err(K32, K64) <= FP32C_TOL
