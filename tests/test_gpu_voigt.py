@@ -305,7 +305,8 @@ def test_device_line_sort_is_stable(E, n):
     assert sp.form == E.HK_FORM_EXOMOL and sp.nlines == n
 
 
-@pytest.mark.parametrize("cm, cut, dnu", [(0, 25.0, 0.001), (1, 400.0, 0.002), (0, 0.0, 0.004)])
+@pytest.mark.parametrize("cm, cut, dnu", [(0, 25.0, 0.001), (1, 400.0, 0.002), (2, 6000.0, 0.001),
+                                          (0, 0.0, 0.004)])
 def test_fp32c_far_field_vs_exact_pairs(E, cm, cut, dnu):
     """The FP32 form's far-field interpolation against the same form with
     every pair evaluated and against the FP64 sum: it must change the
