@@ -40,6 +40,10 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "hk_device.h"
@@ -272,7 +276,9 @@ __device__ __forceinline__ void dispatch64(const Rec64 &q, int code, const doubl
     }
 }
 
-template <int PPT>
+// LM: stages of kThreads * LM lines (each thread stages LM lines per stage);
+// results do not depend on it (lines are added in index order either way).
+template <int PPT, int LM>
 __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *rec) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
@@ -288,36 +294,41 @@ __device__ __forceinline__ void tile_f64(const VArgs &a, int64_t tile, Rec64 *re
         xj[k] = fma((double)(j0 + jr0 + 64 * k), a.dnu, a.numin);
         acc[k] = 0.0;
     }
-    for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
-        const int64_t i = base + t;
-        Rec64 r;
-        r.lo = 0; r.hi = 0; r.code = 0;
-        if (i < ti.hi) {
-            const double nu = a.nu[i];
-            int64_t wl, wh;
-            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
-            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
-            if (clo < chi) {
-                const double ia = a.ia[i], y = a.y[i], y2 = y * y;
-                const double A = a.S[i] * kISqrtPi;
-                r.nu = nu; r.ia = ia; r.y2 = y2;
-                r.B = A * y * kISqrtPi; r.G = A * y * (2.0 / kPi);
-                r.p2 = 3.75 - 2.0 * y2; r.p3 = -15.0 * y2; r.p4 = 12.0 * y2 * y2;
-                r.lo = (int32_t)(clo - j0); r.hi = (int32_t)(chi - j0);
-                const double jcd = rint((nu - a.numin) / a.dnu);
-                const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
-                const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
-                int32_t ri[6];
-                region_interval(kRF64, y2, rr, vb, d0, W, ri[0], ri[1]);
-                region_interval(kRM64, y2, rr, vb, d0, W, ri[2], ri[3]);
-                region_interval(kRB64, y2, rr, vb, d0, W, ri[4], ri[5]);
-                r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, [](int) { return 0.0; }, 1.0, 1.0);
+    for (int64_t base = ti.lo; base < ti.hi; base += kThreads * LM) {
+#pragma unroll 1
+        for (int q = 0; q < LM; ++q) {
+            const int64_t i = base + q * kThreads + t;
+            Rec64 r;
+            r.lo = 0; r.hi = 0; r.code = 0;
+            if (i < ti.hi) {
+                const double nu = a.nu[i];
+                int64_t wl, wh;
+                cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
+                const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
+                if (clo < chi) {
+                    const double ia = a.ia[i], y = a.y[i], y2 = y * y;
+                    const double A = a.S[i] * kISqrtPi;
+                    r.nu = nu; r.ia = ia; r.y2 = y2;
+                    r.B = A * y * kISqrtPi; r.G = A * y * (2.0 / kPi);
+                    r.p2 = 3.75 - 2.0 * y2; r.p3 = -15.0 * y2; r.p4 = 12.0 * y2 * y2;
+                    r.lo = (int32_t)(clo - j0); r.hi = (int32_t)(chi - j0);
+                    const double jcd = rint((nu - a.numin) / a.dnu);
+                    const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
+                    const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
+                    int32_t ri[6];
+                    region_interval(kRF64, y2, rr, vb, d0, W, ri[0], ri[1]);
+                    region_interval(kRM64, y2, rr, vb, d0, W, ri[2], ri[3]);
+                    region_interval(kRB64, y2, rr, vb, d0, W, ri[4], ri[5]);
+                    r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, [](int) { return 0.0; }, 1.0, 1.0);
+                }
             }
+            // the previous stage's readers of rec[] must be done before it is
+            // written (the first record is built while they finish)
+            if (q == 0) __syncthreads();
+            rec[q * kThreads + t] = r;
         }
         __syncthreads();
-        rec[t] = r;
-        __syncthreads();
-        const int nst = (int)min((int64_t)kStage, ti.hi - base);
+        const int nst = (int)min((int64_t)kThreads * LM, ti.hi - base);
         for (int c0 = 0; c0 < nst; c0 += 64) {
             const int nib = c0 + lane < nst ? (rec[c0 + lane].code >> (4 * wave)) & 15 : 0;
             unsigned long long m = __ballot(nib != 0);       // 0: window misses this wave
@@ -454,14 +465,16 @@ __device__ __forceinline__ int compact_slot(bool valid, int *cnt, int &total) {
 // FP64 values at the tile's Chebyshev-Gauss nodes, and the sum over them is
 // interpolated to the tile's points once.  Three bands, by the distance d
 // of the centre from the tile's centre in tile half widths:
-//   band 1, d >= 4:       9 nodes, error <= 4 rho^-8 ~ 2.7e-7 relative
-//                         (Bernstein ellipse rho = 4 + sqrt(15));
-//   band 2, 2 <= d < 4:   15 nodes, error <= 4 rho^-14 ~ 3.9e-8 relative
-//                         (rho = 2 + sqrt(3));
-//   band 3, 1.5 <= d < 2: 19 nodes, error <= 4 rho^-18 ~ 1.2e-7 relative
-//                         (rho = 1.5 + sqrt(1.25));
-// each of a far line's own contribution, which is positive, so the far
-// field's share of K carries at most 2.7e-7.
+//   band 1, d >= 4:       9 nodes;
+//   band 2, 2 <= d < 4:   15 nodes;
+//   band 3, 1.5 <= d < 2: 19 nodes.
+// The worst line of a band has its centre at the band's inner edge and the
+// narrowest profile, 1/(u - d)^2 over u in [-1, 1] (a double pole at the
+// centre); the pointwise relative interpolation error there is 2.2e-7
+// (band 1), 1.4e-7 (band 2) and 9.9e-7 (band 3), largest at the tile end
+// nearest the line.  Far lines contribute positive values, so the far
+// field's share of K carries at most 1e-6 relative.
+// (tests/test_far_field_bounds.py measures these on the constants below.)
 // The node values use the asymptotic series w(z) sqrt(pi) = (i/z) sum_n c_n
 // z^-2n, c_n = (2n-1)!!/2^n, n <= 3, which requires x^2 + y^2 >= kFarR2 over
 // the tile (first omitted term of the real part (105/16) 9 y / x^10 against
@@ -496,11 +509,13 @@ __device__ __forceinline__ double far_u(int k) {
     return N == kFN1 ? kFarU1[k] : (N == kFN2 ? kFarU2[k] : kFarU3[k]);
 }
 
-// One band's node positions, per-wave node sums and coefficients.
+// One band's per-wave node sums (FP32, compensated: hi + lo), their totals
+// and the Chebyshev coefficients.
 template <int N>
 struct FarBand {
-    double xn[N];                     // node positions [cm^-1]
-    double acc[N][kThreads / 64];     // node sums, one column per wave
+    float hi[N][kThreads / 64];       // node sums, one column per wave
+    float lo[N][kThreads / 64];
+    double F[N];                      // node totals
     double c[N];                      // Chebyshev coefficients
 };
 
@@ -512,23 +527,25 @@ struct FarLDS {
     int any1, any2, any3;             // band used (set to 1 by any wave)
 };
 
-// S Re w(x, y) / sqrt(pi) with Bp = S y / pi, from the asymptotic series.
-__device__ __forceinline__ double far_node64(double x, double y) {
-    const double r2 = fma(x, x, y * y);
-    const double ir = rcp64(r2);
-    const double zr = x * ir, zi = -y * ir;               // 1/z
-    const double ur = (zr - zi) * (zr + zi), ui = 2.0 * zr * zi;   // 1/z^2
-    constexpr double c3 = 15.0 / 8.0, c2 = 3.0 / 4.0, c1 = 0.5;
-    double sr = c3, si = 0.0;
+// Re w(x, y) / sqrt(pi) from the asymptotic series in FP32 (x^2 + y^2 >=
+// kFarR2): w sqrt(pi) = (i/z) S, S = sum_n c_n z^-2n, n <= 3.  Every term
+// is below the previous by >= 1/r2 <= 2.5e-3, so the sum carries a few
+// FP32 roundings (< 4e-7 relative).
+__device__ __forceinline__ float far_node32(float x, float y) {
+    const float r2 = fmaf(x, x, y * y);
+    const float ir = rcp32(r2);
+    const float zr = x * ir, zi = -y * ir;                 // 1/z
+    const float ur = (zr - zi) * (zr + zi), ui = 2.0f * zr * zi;   // 1/z^2
+    float sr = 1.875f, si = 0.0f;
 #pragma unroll
     for (int n = 2; n >= 0; --n) {
-        const double cn = n == 2 ? c2 : n == 1 ? c1 : 1.0;
-        const double tr = fma(sr, ur, fma(-si, ui, cn));
-        si = fma(sr, ui, si * ur);
+        const float cn = n == 2 ? 0.75f : n == 1 ? 0.5f : 1.0f;
+        const float tr = fmaf(sr, ur, fmaf(-si, ui, cn));
+        si = fmaf(sr, ui, si * ur);
         sr = tr;
     }
     // Re(i S / z) = -(Im S Re(1/z) + Re S Im(1/z))
-    return -(si * zr + sr * zi) * (1.0 / kPi);
+    return -(si * zr + sr * zi) * (float)(1.0 / kPi);
 }
 
 // DPP move of a double: each lane takes the value of the lane that the
@@ -541,19 +558,26 @@ __device__ __forceinline__ double dpp64(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum of v over the wave, exact in lane 63 (other lanes hold partial sums),
-// in a fixed order: pairs and quads (quad_perm), 8 and 16 lanes (row half
-// mirror, row mirror), then rows 0+1 and 2+3 (row_bcast:15 into rows 1 and
-// 3) and all four (row_bcast:31 into rows 2 and 3).  DPP moves are VALU
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp32(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum of v over the wave, complete in lane 63 (other lanes hold partial
+// sums), in a fixed order: pairs and quads (quad_perm), 8 and 16 lanes (row
+// half mirror, row mirror), then rows 0+1 and 2+3 (row_bcast:15 into rows 1
+// and 3) and all four (row_bcast:31 into rows 2 and 3).  DPP moves are VALU
 // operations; a butterfly of __shfl_xor goes through ds_bpermute instead and
-// waits on the LDS path at every step.
-__device__ __forceinline__ double wave_total64(double v) {
-    v += dpp64<0xb1, 0xf>(v);     // quad_perm [1,0,3,2]
-    v += dpp64<0x4e, 0xf>(v);     // quad_perm [2,3,0,1]
-    v += dpp64<0x141, 0xf>(v);    // row_half_mirror
-    v += dpp64<0x140, 0xf>(v);    // row_mirror
-    v += dpp64<0x142, 0xa>(v);    // row_bcast:15
-    v += dpp64<0x143, 0xc>(v);    // row_bcast:31
+// waits on the LDS path at every step.  For v >= 0 (node values) the six
+// levels of the tree leave the total within 6 roundings (3.6e-7) of exact.
+__device__ __forceinline__ float wave_total32(float v) {
+    v += dpp32<0xb1, 0xf>(v);     // quad_perm [1,0,3,2]
+    v += dpp32<0x4e, 0xf>(v);     // quad_perm [2,3,0,1]
+    v += dpp32<0x141, 0xf>(v);    // row_half_mirror
+    v += dpp32<0x140, 0xf>(v);    // row_mirror
+    v += dpp32<0x142, 0xa>(v);    // row_bcast:15
+    v += dpp32<0x143, 0xc>(v);    // row_bcast:31
     return v;
 }
 
@@ -581,34 +605,48 @@ __device__ __forceinline__ int far_band(double nu, double hw, double ia, double 
 }
 
 template <int N>
-__device__ __forceinline__ void band_init(FarBand<N> *b, int t, int64_t j0, double hW,
-                                          double numin, double dnu) {
-    if (t < N) b->xn[t] = fma((double)j0 + hW * (1.0 + far_u<N>(t)), dnu, numin);
-    if (t < N * (kThreads / 64)) b->acc[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0;
-}
-
-// The wave's lines of this band (mine) add their node values to the wave's
-// column of node sums (all lanes take part in the wave sums).
-template <int N>
-__device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, double fnu, double fia,
-                                           double fy, double fB, int lane, int wave) {
-#pragma unroll 1
-    for (int k = 0; k < N; ++k) {
-        const double v = mine ? fB * far_node64((b->xn[k] - fnu) * fia, fy) : 0.0;
-        const double sw = wave_total64(v);
-        if (lane == 63) b->acc[k][wave] += sw;
+__device__ __forceinline__ void band_init(FarBand<N> *b, int t) {
+    if (t < N * (kThreads / 64)) {
+        b->hi[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0f;
+        b->lo[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0f;
     }
 }
 
-// Node sums of the waves -> Chebyshev coefficients, by threads t0 .. t0 + N - 1
-// (two barriers are taken by the caller between the phases).
+// The wave's lines of this band (mine) add their node values to the wave's
+// column of node sums (all lanes take part in the wave sums).  A line's
+// Doppler-unit offset from the tile centre D and the tile half width h come
+// in FP64 (one per line and tile); each node's x = u_k h + D is rounded to
+// FP32 once, and the series, the strength and the wave sum are FP32.  The
+// column sums over the stages are compensated (two-sum, hi + lo).
+template <int N>
+__device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, double D, double h,
+                                           float fy, float fB, int lane, int wave) {
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        const float x = (float)fma(far_u<N>(k), h, D);
+        const float v = mine ? fB * far_node32(x, fy) : 0.0f;
+        const float sw = wave_total32(v);
+        if (lane == 63) {
+            float hi = b->hi[k][wave], lo = b->lo[k][wave];
+            two_sum_acc(hi, lo, sw);
+            b->hi[k][wave] = hi;
+            b->lo[k][wave] = lo;
+        }
+    }
+}
+
+// Node totals of the waves -> Chebyshev coefficients, by threads t0 .. t0 + N
+// - 1 (two barriers are taken by the caller between the phases).  The
+// interpolation -- totals, coefficients and the per-point Clenshaw sums --
+// runs in FP64: a band-3 tile's far field varies by up to 25x across it, and
+// an FP32 Clenshaw sum would carry up to ~3e-5 of the smallest value.
 template <int N>
 __device__ __forceinline__ void band_sum(FarBand<N> *b, int t) {
     if (t >= 0 && t < N) {
         double F = 0.0;
 #pragma unroll
-        for (int w = 0; w < kThreads / 64; ++w) F += b->acc[t][w];
-        b->acc[t][0] = F;
+        for (int w = 0; w < kThreads / 64; ++w) F += (double)b->hi[t][w] + (double)b->lo[t][w];
+        b->F[t] = F;
     }
 }
 
@@ -625,7 +663,7 @@ __device__ __forceinline__ void band_coeffs(FarBand<N> *b, int t) {
                 tm = tc;
                 tc = tn;
             }
-            c += b->acc[k][0] * (t == 0 ? 1.0 : tc);
+            c += b->F[k] * (t == 0 ? 1.0 : tc);
         }
         b->c[t] = (t == 0 ? 1.0 : 2.0) / (double)N * c;
     }
@@ -644,7 +682,7 @@ __device__ __forceinline__ double band_eval(const FarBand<N> *b, double u) {
     return fma(u, b1, b->c[0] - b2);
 }
 
-template <int PPT>
+template <int PPT, int LM>
 __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec, FarLDS *fl) {
     constexpr int W = kThreads * PPT;
     const int t = threadIdx.x;
@@ -662,40 +700,48 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     const int64_t Wv = min((int64_t)W, a.nx - j0);
     const double hW = 0.5 * (double)(Wv - 1);
     const bool ffok = a.far && Wv >= kFarMinPts;
-    band_init<kFN1>(&fl->b1, t, j0, hW, a.numin, a.dnu);
-    band_init<kFN2>(&fl->b2, t, j0, hW, a.numin, a.dnu);
-    band_init<kFN3>(&fl->b3, t, j0, hW, a.numin, a.dnu);
+    band_init<kFN1>(&fl->b1, t);
+    band_init<kFN2>(&fl->b2, t);
+    band_init<kFN3>(&fl->b3, t);
+    const double xc = fma((double)j0 + hW, a.dnu, a.numin);     // tile centre [cm^-1]
     if (t == 0) { fl->any1 = 0; fl->any2 = 0; fl->any3 = 0; }
     __syncthreads();
     int nprev = 0, par = 0;
-    for (int64_t base = ti.lo; base < ti.hi; base += kStage) {
-        const int64_t i = base + t;
+    for (int64_t base = ti.lo; base < ti.hi; base += kThreads * LM) {
+      int nst = 0;
+#pragma unroll 1
+      for (int q = 0; q < LM; ++q) {
+        const int64_t i = base + q * kThreads + t;
         Rec32 r;
         r.lo = 0; r.hi = 0; r.code = 0;
         bool far = false;
-        // far lines first: FP64 values at the nodes, summed per wave (their
+        // far lines first: FP32 values at the nodes, summed per wave (their
         // registers are free again before the direct records are built)
         if (ffok) {
-            double fy = 0.0, fnu = 0.0, fia = 0.0, fB = 0.0;
+            double fD = 0.0, fh = 0.0;
+            float fy = 0.0f, fB = 0.0f;
             int band = 0;
             if (i < ti.hi) {
-                fnu = a.nu[i];
-                fia = a.ia[i];
-                fy = a.y[i];
-                band = far_band(fnu, a.hw[i], fia, fy, j0, Wv, hW, a.numin, a.dnu);
-                if (band) fB = ldexp(a.S[i], -ti.e);
+                const double fnu = a.nu[i], fia = a.ia[i], y = a.y[i];
+                band = far_band(fnu, a.hw[i], fia, y, j0, Wv, hW, a.numin, a.dnu);
+                if (band) {
+                    fD = (xc - fnu) * fia;
+                    fh = hW * a.dnu * fia;
+                    fy = (float)y;
+                    fB = (float)ldexp(a.S[i], -ti.e);
+                }
             }
             far = band != 0;
             if (__ballot(band == 1) != 0ull) {
-                band_nodes<kFN1>(&fl->b1, band == 1, fnu, fia, fy, fB, lane, wave);
+                band_nodes<kFN1>(&fl->b1, band == 1, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any1 = 1;
             }
             if (__ballot(band == 2) != 0ull) {
-                band_nodes<kFN2>(&fl->b2, band == 2, fnu, fia, fy, fB, lane, wave);
+                band_nodes<kFN2>(&fl->b2, band == 2, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any2 = 1;
             }
             if (__ballot(band == 3) != 0ull) {
-                band_nodes<kFN3>(&fl->b3, band == 3, fnu, fia, fy, fB, lane, wave);
+                band_nodes<kFN3>(&fl->b3, band == 3, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any3 = 1;
             }
         }
@@ -757,11 +803,13 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         // compact_slot.  The counts alternate between two buffers, so a wave
         // that skips ahead cannot overwrite counts another wave has not read
         // (it cannot pass the next compact_slot barrier before that wave).
-        if (nprev > 0) __syncthreads();
-        int nst;
-        const int slot = compact_slot(r.code != 0, fl->cnt[par], nst);
+        if (q == 0 && nprev > 0) __syncthreads();
+        int nq;
+        const int slot = compact_slot(r.code != 0, fl->cnt[par], nq);
         par ^= 1;
-        if (r.code != 0) rec[slot] = r;
+        if (r.code != 0) rec[nst + slot] = r;
+        nst += nq;
+      }
         if (nst > 0) __syncthreads();
         nprev = nst;
         for (int c0 = 0; c0 < nst; c0 += 64) {
@@ -966,16 +1014,16 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
 // XCDs balanced; giving each XCD a contiguous range instead (better L2 reuse
 // of line data) measured 1.1x slower for the main pass and 2x for the
 // line-core pass.
-template <int MODE, int PPT>
+template <int MODE, int PPT, int LM>
 __global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
     const int64_t tile = blockIdx.x;
     if (MODE == HK_VOIGT_FP64) {
-        __shared__ Rec64 rec[kStage];
-        tile_f64<PPT>(a, tile, rec);
+        __shared__ Rec64 rec[kThreads * LM];
+        tile_f64<PPT, LM>(a, tile, rec);
     } else {
-        __shared__ Rec32 rec[kStage];
+        __shared__ Rec32 rec[kThreads * LM];
         __shared__ FarLDS fl;
-        tile_f32c<PPT>(a, tile, rec, &fl);
+        tile_f32c<PPT, LM>(a, tile, rec, &fl);
     }
 }
 
@@ -997,11 +1045,12 @@ __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
 // item_start[s] = first item of spectrum s (nspec+1 entries).  PPT == 0
 // selects the line-core pass.
 // Occupancy floors: the main pass 4 waves per SIMD for tiles up to 1024
-// points, 3 for 2048 (without them the far-field code raises the register
-// count of the persistent form enough to halve its occupancy), the FP32
-// line-core pass 5 (as its one-spectrum kernel has without a floor).
-template <int MODE, int PPT>
-__global__ __launch_bounds__(256, PPT == 0 ? (MODE == HK_VOIGT_FP32C ? 5 : 1) : (PPT <= 4 ? 4 : (PPT == 8 ? 3 : 1)))
+// points (3 with 512-line stages, whose LDS allows 3 workgroups per CU), 3
+// for 2048 (without them the far-field code raises the register count of
+// the persistent form enough to halve its occupancy), the FP32 line-core
+// pass 5 (as its one-spectrum kernel has without a floor).
+template <int MODE, int PPT, int LM>
+__global__ __launch_bounds__(256, PPT == 0 ? (MODE == HK_VOIGT_FP32C ? 5 : 1) : (PPT <= 4 ? (LM == 2 ? 3 : 4) : (PPT == 8 ? 3 : 1)))
 void voigt_persistent_kernel(const VArgs *__restrict__ specs,
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
@@ -1011,7 +1060,7 @@ void voigt_persistent_kernel(const VArgs *__restrict__ specs,
     __shared__ int64_t s_item;
     __shared__ int cnt[kThreads / 64];
     __shared__ float dtab[(PPT == 0 && MODE == HK_VOIGT_FP32C) ? kStage * kDT : 1];
-    __shared__ __attribute__((aligned(16))) char smem[kRecBytes * kStage];
+    __shared__ __attribute__((aligned(16))) char smem[kRecBytes * kThreads * LM];
     __shared__ FarLDS fl[1];
     const int64_t total = item_start[nspec];
     for (;;) {
@@ -1032,8 +1081,8 @@ void voigt_persistent_kernel(const VArgs *__restrict__ specs,
             if constexpr (MODE == HK_VOIGT_FP64) core_f64(a, tile, (Core64 *)smem, cnt);
             else core_f32c(a, tile, (Core32 *)smem, dtab, cnt);
         } else {
-            if constexpr (MODE == HK_VOIGT_FP64) tile_f64<PPT>(a, tile, (Rec64 *)smem);
-            else tile_f32c<PPT>(a, tile, (Rec32 *)smem, fl);
+            if constexpr (MODE == HK_VOIGT_FP64) tile_f64<PPT, LM>(a, tile, (Rec64 *)smem);
+            else tile_f32c<PPT, LM>(a, tile, (Rec32 *)smem, fl);
         }
     }
 }
@@ -1166,28 +1215,53 @@ int pick_ppt(int mode, const hk_tune *tune) {
 
 bool ppt_ok(int ppt) { return ppt == 2 || ppt == 4 || ppt == 8 || ppt == 16; }
 
-template <int MODE>
-void launch_tiles(int ppt, int64_t ntiles, hipStream_t s, const VArgs &a) {
+// Lines per LDS stage of the main pass: 256 (default) or 512; returns the
+// multiple of kThreads, 0 when the request is not one of those.
+int pick_lm(const hk_tune *tune) {
+    const int sl = tune && tune->stage_lines > 0 ? tune->stage_lines : kThreads;
+    return sl == kThreads ? 1 : (sl == 2 * kThreads ? 2 : 0);
+}
+
+template <int MODE, int LM>
+void launch_tiles_lm(int ppt, int64_t ntiles, hipStream_t s, const VArgs &a) {
     dim3 g((unsigned)ntiles), b(kThreads);
     switch (ppt) {
-        case 2: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 2>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 4>), g, b, 0, s, a); break;
-        case 8: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 8>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 16>), g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 2, LM>), g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 4, LM>), g, b, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 8, LM>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((voigt_tiles_kernel<MODE, 16, LM>), g, b, 0, s, a); break;
     }
 }
 
 template <int MODE>
-void launch_persistent(int ppt, unsigned grid, hipStream_t s, const VArgs *specs,
-                       const int64_t *item_start, int nspec, unsigned long long *counter) {
+void launch_tiles(int ppt, int lm, int64_t ntiles, hipStream_t s, const VArgs &a) {
+    if (lm == 2) launch_tiles_lm<MODE, 2>(ppt, ntiles, s, a);
+    else launch_tiles_lm<MODE, 1>(ppt, ntiles, s, a);
+}
+
+template <int MODE, int LM>
+void launch_persistent_lm(int ppt, unsigned grid, hipStream_t s, const VArgs *specs,
+                          const int64_t *item_start, int nspec, unsigned long long *counter) {
     dim3 g(grid), b(kThreads);
     switch (ppt) {
-        case 0: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 0>), g, b, 0, s, specs, item_start, nspec, counter); break;
-        case 2: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 2>), g, b, 0, s, specs, item_start, nspec, counter); break;
-        case 4: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 4>), g, b, 0, s, specs, item_start, nspec, counter); break;
-        case 8: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 8>), g, b, 0, s, specs, item_start, nspec, counter); break;
-        default: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 16>), g, b, 0, s, specs, item_start, nspec, counter); break;
+        case 2: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 2, LM>), g, b, 0, s, specs, item_start, nspec, counter); break;
+        case 4: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 4, LM>), g, b, 0, s, specs, item_start, nspec, counter); break;
+        case 8: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 8, LM>), g, b, 0, s, specs, item_start, nspec, counter); break;
+        default: hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 16, LM>), g, b, 0, s, specs, item_start, nspec, counter); break;
     }
+}
+
+// ppt == 0: the line-core pass (its stages are kThreads lines)
+template <int MODE>
+void launch_persistent(int ppt, int lm, unsigned grid, hipStream_t s, const VArgs *specs,
+                       const int64_t *item_start, int nspec, unsigned long long *counter) {
+    if (ppt == 0)
+        hipLaunchKernelGGL((voigt_persistent_kernel<MODE, 0, 1>), dim3(grid), dim3(kThreads), 0, s,
+                           specs, item_start, nspec, counter);
+    else if (lm == 2)
+        launch_persistent_lm<MODE, 2>(ppt, grid, s, specs, item_start, nspec, counter);
+    else
+        launch_persistent_lm<MODE, 1>(ppt, grid, s, specs, item_start, nspec, counter);
 }
 
 int check_state(const hk_line_state *st, const char *fn) {
@@ -1225,6 +1299,8 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
     if (int e = check_grid(grid, "hk_voigt_accumulate")) return e;
     const int ppt = pick_ppt(mode, tune);
     HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_accumulate: tile_points must be 512, 1024, 2048 or 4096");
+    const int lm = pick_lm(tune);
+    HK_CHECK_ARG(lm > 0, "hk_voigt_accumulate: stage_lines must be 256 or 512");
     if (ws_bytes < hk_voigt_workspace_bytes(grid, tune)) {
         set_error("hk_voigt_accumulate: workspace %zu bytes, %zu needed", ws_bytes,
                   hk_voigt_workspace_bytes(grid, tune));
@@ -1253,8 +1329,8 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
     a.tiles = tiles; a.numin = grid->numin; a.dnu = grid->dnu; a.nx = grid->nx;
     a.kmin = kmin; a.accumulate = accumulate; a.K = K;
     a.far = (mode == HK_VOIGT_FP32C && !(tune && tune->exact_pairs)) ? 1 : 0;
-    if (mode == HK_VOIGT_FP64) launch_tiles<HK_VOIGT_FP64>(ppt, ntiles, s, a);
-    else launch_tiles<HK_VOIGT_FP32C>(ppt, ntiles, s, a);
+    if (mode == HK_VOIGT_FP64) launch_tiles<HK_VOIGT_FP64>(ppt, lm, ntiles, s, a);
+    else launch_tiles<HK_VOIGT_FP32C>(ppt, lm, ntiles, s, a);
     HK_LAUNCH_CHECK();
     a.tiles = ctiles;
     if (mode == HK_VOIGT_FP64)
@@ -1328,6 +1404,8 @@ extern "C" int hk_voigt_batched_prepare(int nspec, const hk_spectrum *spec, int 
                  "hk_voigt_batched_prepare: mode %d", mode);
     const int ppt = pick_ppt(mode, tune);
     HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_batched_prepare: tile_points must be 512, 1024, 2048 or 4096");
+    const int lm = pick_lm(tune);
+    HK_CHECK_ARG(lm > 0, "hk_voigt_batched_prepare: stage_lines must be 256 or 512");
     for (int i = 0; i < nspec; ++i) {
         if (int e = check_state(&spec[i].st, "hk_voigt_batched_prepare")) return e;
         if (int e = check_grid(&spec[i].grid, "hk_voigt_batched_prepare")) return e;
@@ -1383,7 +1461,7 @@ extern "C" int hk_voigt_batched_prepare(int nspec, const hk_spectrum *spec, int 
     batch->nspec = nspec;
     batch->mode = mode;
     batch->tile_points = W;
-    batch->pad = 0;
+    batch->stage_lines = kThreads * lm;
     batch->items_main = items[nspec];
     batch->items_core = citems[nspec];
     batch->off_args = L.off_args; batch->off_cargs = L.off_cargs;
@@ -1396,7 +1474,9 @@ extern "C" int hk_voigt_batched_launch(const hk_voigt_batch *batch, hk_stream_t 
     HK_CHECK_ARG(batch && batch->workspace && batch->nspec > 0,
                  "hk_voigt_batched_launch: batch not prepared");
     const int ppt = batch->tile_points / kThreads;
-    HK_CHECK_ARG(ppt_ok(ppt) && (batch->mode == HK_VOIGT_FP64 || batch->mode == HK_VOIGT_FP32C),
+    const int lm = batch->stage_lines / kThreads;
+    HK_CHECK_ARG(ppt_ok(ppt) && (lm == 1 || lm == 2) &&
+                     (batch->mode == HK_VOIGT_FP64 || batch->mode == HK_VOIGT_FP32C),
                  "hk_voigt_batched_launch: batch not prepared");
     char *base = (char *)batch->workspace;
     const VArgs *args = (const VArgs *)(base + batch->off_args);
@@ -1419,13 +1499,13 @@ extern "C" int hk_voigt_batched_launch(const hk_voigt_batch *batch, hk_stream_t 
         const unsigned gm = persistent_grid(batch->items_main);
         const unsigned gc = persistent_grid(batch->items_core);
         if (batch->mode == HK_VOIGT_FP64) {
-            launch_persistent<HK_VOIGT_FP64>(ppt, gm, s, args, items, nspec, counter);
+            launch_persistent<HK_VOIGT_FP64>(ppt, lm, gm, s, args, items, nspec, counter);
             HK_LAUNCH_CHECK();
-            launch_persistent<HK_VOIGT_FP64>(0, gc, s, cargs, citems, nspec, counter + 1);
+            launch_persistent<HK_VOIGT_FP64>(0, 1, gc, s, cargs, citems, nspec, counter + 1);
         } else {
-            launch_persistent<HK_VOIGT_FP32C>(ppt, gm, s, args, items, nspec, counter);
+            launch_persistent<HK_VOIGT_FP32C>(ppt, lm, gm, s, args, items, nspec, counter);
             HK_LAUNCH_CHECK();
-            launch_persistent<HK_VOIGT_FP32C>(0, gc, s, cargs, citems, nspec, counter + 1);
+            launch_persistent<HK_VOIGT_FP32C>(0, 1, gc, s, cargs, citems, nspec, counter + 1);
         }
         HK_LAUNCH_CHECK();
     }
@@ -1440,20 +1520,84 @@ extern "C" int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode, co
     return hk_voigt_batched_launch(&b, stream);
 }
 
+// Tuning cache: the choice of the first tuning of a problem is kept for the
+// process, so reruns of the same problem take the same tile width (and give
+// the same bits).  The signature: form, far-field setting, grid, line count,
+// and the line population's cut reach and median y (y of 64 lines spread
+// over the list), both rounded to 3 significant digits.
+namespace {
+struct TuneKey {
+    int mode, exact;
+    int64_t nx, nl;
+    double numin, dnu, reach, ymed;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(mode, exact, nx, nl, numin, dnu, reach, ymed) <
+               std::tie(o.mode, o.exact, o.nx, o.nl, o.numin, o.dnu, o.reach, o.ymed);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;     // -> (tile points, stage lines)
+
+double round3(double v) {
+    if (!(v != 0.0) || !std::isfinite(v)) return v;
+    const double e = std::pow(10.0, std::floor(std::log10(std::fabs(v))) - 2.0);
+    return std::round(v / e) * e;
+}
+
+constexpr int kTuneSamples = 64;
+
+__global__ void tune_sample_kernel(const double *__restrict__ y, int64_t nl,
+                                   const double *__restrict__ radius, double *__restrict__ out) {
+    const int t = threadIdx.x;
+    if (t < kTuneSamples) out[t] = nl > 0 ? y[(nl - 1) * t / (kTuneSamples - 1)] : 0.0;
+    if (t == 0) out[kTuneSamples] = radius[0];
+}
+}  // namespace
+
+extern "C" void hk_voigt_autotune_reset(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
 extern "C" int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, int mode,
                                  double *K, void *ws, size_t ws_bytes, hk_tune *best, float *ms,
                                  hk_stream_t stream) {
     HK_CHECK_ARG(st && grid && K && ws && best, "hk_voigt_autotune: null argument");
+    if (int e = check_state(st, "hk_voigt_autotune")) return e;
+    if (int e = check_grid(grid, "hk_voigt_autotune")) return e;
     constexpr int kCand[4] = {512, 1024, 2048, 4096};
+    constexpr int kStageCand[2] = {256, 512};
     hipStream_t s = (hipStream_t)stream;
+    const int exact = best->exact_pairs;     // tuned for the caller's setting
+    // signature (K serves as the scratch of the samples: it is overwritten anyway)
+    TuneKey key = {mode, exact, grid->nx, st->nlines, grid->numin, grid->dnu, 0.0, 0.0};
+    const bool cacheable = grid->nx > kTuneSamples;
+    if (cacheable) {
+        hipLaunchKernelGGL(tune_sample_kernel, dim3(1), dim3(kTuneSamples), 0, s, st->y,
+                           st->nlines, st->radius, K);
+        HK_LAUNCH_CHECK();
+        double h[kTuneSamples + 1];
+        HK_HIP(hipMemcpyAsync(h, K, sizeof(h), hipMemcpyDeviceToHost, s));
+        HK_HIP(hipStreamSynchronize(s));
+        std::sort(h, h + kTuneSamples);
+        key.ymed = round3(0.5 * (h[kTuneSamples / 2 - 1] + h[kTuneSamples / 2]));
+        key.reach = round3(h[kTuneSamples]);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            best->tile_points = it->second.first;
+            best->stage_lines = it->second.second;
+            if (ms) for (int c = 0; c < 8; ++c) ms[c] = 0.0f;
+            return HK_OK;
+        }
+    }
     hipEvent_t e0, e1;
     HK_HIP(hipEventCreate(&e0));
     HK_HIP(hipEventCreate(&e1));
     float bestms = 0.0f;
-    int bestw = 0, rc = HK_OK;
-    const int exact = best->exact_pairs;     // tuned for the caller's setting
-    for (int c = 0; c < 4 && rc == HK_OK; ++c) {
-        hk_tune t = {kCand[c], exact};
+    int bestw = 0, bests = 0, rc = HK_OK;
+    for (int c = 0; c < 8 && rc == HK_OK; ++c) {
+        hk_tune t = {kCand[c % 4], exact, kStageCand[c / 4]};
         rc = hk_voigt_accumulate(st, grid, 0.0, 0, mode, &t, K, ws, ws_bytes, stream);
         if (rc != HK_OK) break;
         if (hipEventRecord(e0, s) != hipSuccess) { rc = HK_ERR_HIP; break; }
@@ -1467,11 +1611,18 @@ extern "C" int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, i
             break;
         }
         if (ms) ms[c] = dt;
-        if (bestw == 0 || dt < bestms) { bestms = dt; bestw = kCand[c]; }
+        if (bestw == 0 || dt < bestms) { bestms = dt; bestw = t.tile_points; bests = t.stage_lines; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc == HK_OK) best->tile_points = bestw;
+    if (rc == HK_OK) {
+        best->tile_points = bestw;
+        best->stage_lines = bests;
+        if (cacheable) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned.emplace(key, std::make_pair(bestw, bests));
+        }
+    }
     return rc;
 }
 

@@ -391,57 +391,87 @@ int database_doubles(int db) {
 
 // Partition function at T: linear interpolation in a 2- or 3-column table
 // (host.h:9-73); T must lie inside the table.
+// Partition function table of one isotopologue: rows "T Q" (npfcol 2) or
+// "T Q x" (npfcol 3), at most 100000, in ascending T.  read_rows stops at the
+// first row that does not parse (end of file included).
+struct PfTable {
+    std::vector<double> T, Q;
+    bool short_read = false;     // a read failed before the row limit
+};
+
+static bool read_pf_rows(FILE *f, int ncol, PfTable &t) {
+    constexpr int kMaxRows = 100000;
+    for (int j = 0; j < kMaxRows; ++j) {
+        double a, b, c;
+        const int got = ncol == 3 ? std::fscanf(f, "%lf %lf %lf", &a, &b, &c)
+                                  : std::fscanf(f, "%lf %lf", &a, &b);
+        if (got != ncol) { t.short_read = true; return true; }
+        t.T.push_back(a);
+        t.Q.push_back(b);
+    }
+    return true;
+}
+
+// Q(T) from one table with readPartition's rules and messages (host.h:9-73):
+// T below the first row is an error; the first row at T gives its Q; else Q
+// is linear between the rows T0 < T <= T1; a repeated temperature below T
+// (which is also how a read past the last row shows) means T is not covered;
+// a table that is exhausted without covering T is incomplete.
+static bool pf_value(const PfTable &t, double T, double &q) {
+    const size_t n = t.T.size();
+    if (n > 0 && t.T[0] > T) {
+        std::printf("Error: partition function not valid for given temperature. T1: %g, T: %g\n",
+                    t.T[0], T);
+        return false;
+    }
+    if (n == 0) {
+        std::printf("Error: partition function can not be read\n");
+        return false;
+    }
+    if (t.T[0] >= T) { q = t.Q[0]; return true; }
+    // rows j >= 1; a failed read repeats the last row (T0 == T1)
+    const size_t steps = t.short_read ? n + 1 : n;
+    for (size_t j = 1; j < steps; ++j) {
+        const double T0 = t.T[j - 1], q0 = t.Q[j - 1];
+        const bool have = j < n;
+        const double T1 = have ? t.T[j] : T0, q1 = have ? t.Q[j] : q0;
+        if (T0 == T1 && T0 < T) {
+            std::printf("Error: partition function not valid for given temperature. T0: %g, T: %g\n",
+                        T0, T);
+            return false;
+        }
+        if (!have) {
+            std::printf("Error: partition function can not be read\n");
+            return false;
+        }
+        if (T0 < T && T1 >= T) {
+            q = (q1 - q0) * ((T - T0) / (T1 - T0)) + q0;
+            return true;
+        }
+    }
+    std::printf("Error: partition function not complete\n");
+    return false;
+}
+
 bool read_partition(const Molecule &m, double T, std::vector<double> &Q) {
     Q.assign(m.iso.size(), 1.0);
     for (size_t i = 0; i < m.iso.size(); ++i) {
-        FILE *f = std::fopen(m.iso[i].qfile.c_str(), "r");
-        std::printf("Read partition function: %s\n", m.iso[i].qfile.c_str());
+        const std::string &name = m.iso[i].qfile;
+        std::printf("Read partition function: %s\n", name.c_str());
+        FILE *f = std::fopen(name.c_str(), "r");
         if (!f) {
-            std::printf("Error: partition file not found %s\n", m.iso[i].qfile.c_str());
+            std::printf("Error: partition file not found %s\n", name.c_str());
             return false;
         }
-        double T0 = -1.0, T1 = -1.0, q0 = 1.0, q1 = 1.0, q = 1.0, skip;
-        bool done = false;
-        for (int j = 0; j < 100000; ++j) {
-            T0 = T1;
-            q0 = q1;
-            int er;
-            if (m.npfcol == 3) er = std::fscanf(f, "%lf %lf %lf", &T1, &q1, &skip) == 3;
-            else if (m.npfcol == 2) er = std::fscanf(f, "%lf %lf", &T1, &q1) == 2;
-            else {
-                std::printf("Error: partition file not specified\n");
-                std::fclose(f);
-                return false;
-            }
-            if (T0 == T1 && T0 < T && j > 0) {
-                std::printf("Error: partition function not valid for given temperature. T0: %g, T: %g\n", T0, T);
-                std::fclose(f);
-                return false;
-            }
-            if (T1 > T && j == 0) {
-                std::printf("Error: partition function not valid for given temperature. T1: %g, T: %g\n", T1, T);
-                std::fclose(f);
-                return false;
-            }
-            if (!er) {
-                std::printf("Error: partition function can not be read\n");
-                std::fclose(f);
-                return false;
-            }
-            if (T0 < T && T1 >= T) {
-                const double tt = (T - T0) / (T1 - T0);
-                q = (q1 - q0) * tt + q0;
-                if (j == 0) q = q1;
-                done = true;
-                break;
-            }
+        if (m.npfcol != 2 && m.npfcol != 3) {
+            std::printf("Error: partition file not specified\n");
+            std::fclose(f);
+            return false;
         }
+        PfTable t;
+        read_pf_rows(f, m.npfcol, t);
         std::fclose(f);
-        if (!done) {
-            std::printf("Error: partition function not complete\n");
-            return false;
-        }
-        Q[i] = q;
+        if (!pf_value(t, T, Q[i])) return false;
     }
     return true;
 }
@@ -657,68 +687,91 @@ bool init_cia(const Param &p, CiaSystem &c) {
     return fail("Error: cia System not found %s", p.ciaSystem.c_str());
 }
 
-// K from a HITRAN CIA table at (T, P) on the points x (readCiaFile,
-// host.h:670-789): the sets are read in file order; within a set K is linear
-// in nu between the table points; a set with T_set <= T replaces K, a set
-// with T < T_set blends linearly in T with the previous set's K; reading
-// stops at the first set with T <= T_set.  Then K = K rho N_A / mass1 + kmin
-// with rho = P 273.15/T amagat (cm^-3), or kmin where T lies above every
-// set.  The fixed-width header fields (20, 10, 10, 7, 7, 10, 36 characters)
-// are read as the reference reads them.
+// One temperature set of a HITRAN collision-induced absorption table: a
+// 100-character header line -- chemical symbol (columns 0-19), wavenumber
+// range (20-29, 30-39), number of points (40-46), temperature (47-53), ... --
+// then one "nu absorption" row per point, nu ascending.
+struct CiaSet {
+    double nu_hi = 0.0;          // upper wavenumber of the header
+    double T = 0.0;
+    std::vector<double> nu, a;
+};
+
+static bool read_cia_set(FILE *f, CiaSet &c) {
+    char line[512];
+    if (!std::fgets(line, sizeof line, f)) return false;
+    const std::string h(line);
+    if (h.size() < 54) return false;
+    c.nu_hi = std::strtod(h.substr(30, 10).c_str(), nullptr);
+    const int n = std::atoi(h.substr(40, 7).c_str());
+    c.T = std::strtod(h.substr(47, 7).c_str(), nullptr);
+    if (n < 2) return false;
+    c.nu.resize(n);
+    c.a.resize(n);
+    for (int k = 0; k < n; ++k)
+        if (std::fscanf(f, "%lf %lf", &c.nu[k], &c.a[k]) != 2) return false;
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF && ch != '\n') {}     // rest of the last row
+    return true;
+}
+
+// The set on the ascending points x, by readCiaFile's sweep (host.h:705-760):
+// a point inside a table interval [nu_j, nu_j+1) takes the linear
+// interpolation; points below the table or at or above its last wavenumber
+// take nothing; and when the last table wavenumber is the header's upper
+// limit the sweep ends at the first point at or above the second-to-last
+// table wavenumber, so of the last interval only that first point is
+// filled.
+static void cia_set_on_points(const CiaSet &c, const std::vector<double> &x, std::vector<double> &v,
+                              std::vector<char> &has) {
+    const int n = (int)c.nu.size();
+    const bool ends = c.nu[n - 1] == c.nu_hi;
+    std::fill(has.begin(), has.end(), 0);
+    int j = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        while (j < n - 2 && x[i] >= c.nu[j + 1]) ++j;
+        if (c.nu[j] <= x[i] && x[i] < c.nu[j + 1]) {
+            has[i] = 1;
+            v[i] = c.a[j] + (c.a[j + 1] - c.a[j]) * (x[i] - c.nu[j]) / (c.nu[j + 1] - c.nu[j]);
+        }
+        if (ends && j == n - 2) break;
+    }
+}
+
+// K at (T, P) on the points x from the sets in file order (readCiaFile,
+// host.h:670-789): a set at T_s <= T replaces K where it has values; the
+// first set above T blends linearly in T with the previous set,
+// K + (k_s - K)(T - T_prev)/(T_s - T_prev) (nothing when it is the first
+// set); no set is read after it.  Then K = K rho N_A / mass1 + kmin with
+// rho = P 273.15/T amagat, or kmin everywhere when T lies above every set.
 bool read_cia(const Param &p, const CiaSystem &cia, const std::vector<double> &x, double P,
               std::vector<double> &K) {
     FILE *f = std::fopen(cia.file.c_str(), "r");
     if (!f) return fail("Error: cia file is not available: %s", cia.file.c_str());
-    const int64_t Nx = (int64_t)x.size();
-    K.assign(Nx, 0.0);
     const double T = p.T;
-    double Tc0, Tc1 = 0.0;
-    char skip[160], c1[12], c2[12], c3[9], c4[9], c5[12];
+    K.assign(x.size(), 0.0);
+    std::vector<double> v(x.size());
+    std::vector<char> has(x.size());
+    double Tprev = 0.0, Tlast = 0.0;
     for (int it = 0; it < cia.nsets; ++it) {
-        Tc0 = Tc1;
-        if (!std::fgets(skip, 21, f) || !std::fgets(c1, 11, f) || !std::fgets(c2, 11, f) ||
-            !std::fgets(c3, 8, f) || !std::fgets(c4, 8, f) || !std::fgets(c5, 11, f) ||
-            !std::fgets(skip, 37, f)) {
+        CiaSet c;
+        if (!read_cia_set(f, c)) {
             std::fclose(f);
             return fail("Error: cia file too short: %s", cia.file.c_str());
         }
-        const int ncia = std::atoi(c3);
-        Tc1 = std::strtod(c4, nullptr);
-        const double numax = std::strtod(c2, nullptr);
-        double nu0, a0, nu1, a1;
-        if (std::fscanf(f, "%lf %lf %lf %lf", &nu0, &a0, &nu1, &a1) != 4) {
-            std::fclose(f);
-            return fail("Error: cia file too short: %s", cia.file.c_str());
+        cia_set_on_points(c, x, v, has);
+        for (size_t i = 0; i < x.size(); ++i) {
+            if (!has[i]) continue;
+            if (c.T <= T) K[i] = v[i];
+            else if (Tprev > 0.0) K[i] = K[i] + (v[i] - K[i]) * (T - Tprev) / (c.T - Tprev);
         }
-        int nc = 2;
-        for (int64_t i = 0; i < Nx; ++i) {
-            while (x[i] >= nu1) {
-                nu0 = nu1;
-                a0 = a1;
-                if (std::fscanf(f, "%lf %lf", &nu1, &a1) != 2) break;
-                ++nc;
-                if (nc == ncia) break;
-            }
-            if (nu0 <= x[i] && x[i] < nu1) {
-                const double k1 = a0 + (a1 - a0) * (x[i] - nu0) / (nu1 - nu0);
-                if (Tc1 <= T) K[i] = k1;
-                if (T < Tc1 && Tc0 > 0.0) K[i] = K[i] + (k1 - K[i]) * (T - Tc0) / (Tc1 - Tc0);
-            }
-            if (nu1 == numax) break;
-            if (i == Nx - 1) {
-                while (nc < ncia) {
-                    if (std::fscanf(f, "%lf %lf", &nu1, &a1) != 2) break;
-                    ++nc;
-                }
-            }
-        }
-        if (!std::fgets(skip, 3, f)) { /* end of file after the last set */ }
-        if (T <= Tc1) break;
+        Tprev = Tlast = c.T;
+        if (T <= c.T) break;
     }
     std::fclose(f);
     const double rho = P * 273.15 / T * 2.6867774e19;     // define.h:43, molecules cm^-3
-    for (int64_t i = 0; i < Nx; ++i)
-        K[i] = Tc1 < T ? p.kmin : K[i] * rho * (kNA / cia.mass1) + p.kmin;
+    for (size_t i = 0; i < x.size(); ++i)
+        K[i] = Tlast < T ? p.kmin : K[i] * rho * (kNA / cia.mass1) + p.kmin;
     return true;
 }
 
@@ -1060,31 +1113,44 @@ bool write_bins_file_K(FILE *kb, const std::string &kname, const Param &p, const
     return true;
 }
 
-// Sub-Lorentzian file (host.h:489-519): sigma1..3, a header, B1..3 rows of
-// alpha beta epsilon, "range in cm^-1" and the range; values are read as
-// float, as there, and B_i = alpha_i + beta_i exp(-epsilon_i T) rounded to
-// float (subLorentzianB, voigt.h:12-35).
+// Sub-Lorentzian file (host.h:489-519), whitespace-separated tokens:
+//   sigma1 v  sigma2 v  sigma3 v        (name, value) x 3
+//   4 header words
+//   B1 alpha beta epsilon  B2 ...  B3 ...
+//   range in cm^-1  lo hi
+// Values are read as float, as there, and B_i = alpha_i + beta_i
+// exp(-epsilon_i T) is rounded to float (subLorentzianB, voigt.h:12-35).
 bool read_sublorentz(const Param &p, hk_chi &c) {
     std::printf("Use sub-Lorentzian file %s\n", p.subLorentzian.c_str());
     FILE *f = std::fopen(p.subLorentzian.c_str(), "r");
     if (!f) return fail("Error: sub-Lorentzian file not found: %s", p.subLorentzian.c_str());
+    std::vector<std::string> tok;
     char b[160];
-    float v[14];
-    int ok = 1;
-    for (int i = 0; i < 3; ++i) ok &= std::fscanf(f, "%159s %f", b, &v[i]) == 2;
-    ok &= std::fscanf(f, "%159s %159s %159s %159s", b, b, b, b) == 4;
-    for (int i = 0; i < 3; ++i)
-        ok &= std::fscanf(f, "%159s %f %f %f", b, &v[3 + 3 * i], &v[4 + 3 * i], &v[5 + 3 * i]) == 4;
-    ok &= std::fscanf(f, "%159s %159s %159s", b, b, b) == 3;
-    ok &= std::fscanf(f, "%f %f", &v[12], &v[13]) == 2;
+    while (tok.size() < 27 && std::fscanf(f, "%159s", b) == 1) tok.push_back(b);
     std::fclose(f);
+    // token positions of the numbers
+    static const int kSigma[3] = {1, 3, 5};
+    static const int kRow[3] = {11, 15, 19};     // alpha of B1..B3; beta, epsilon follow
+    static const int kRange[2] = {25, 26};
+    auto num = [&](int k, float &out) {
+        if (k >= (int)tok.size()) return false;
+        char *end = nullptr;
+        out = std::strtof(tok[k].c_str(), &end);
+        return end != tok[k].c_str();
+    };
+    float sig[3], al[3], be[3], ep[3], rg[2];
+    bool ok = tok.size() == 27;
+    for (int i = 0; i < 3 && ok; ++i)
+        ok = num(kSigma[i], sig[i]) && num(kRow[i], al[i]) && num(kRow[i] + 1, be[i]) &&
+             num(kRow[i] + 2, ep[i]);
+    ok = ok && num(kRange[0], rg[0]) && num(kRange[1], rg[1]);
     if (!ok) return fail("Error in sub-Lorentzian file read %s", p.subLorentzian.c_str());
     for (int i = 0; i < 3; ++i) {
-        c.sigma[i] = v[i];
-        c.B[i] = (float)(v[3 + 3 * i] + v[4 + 3 * i] * std::exp(-v[5 + 3 * i] * p.T));
+        c.sigma[i] = sig[i];
+        c.B[i] = (float)(al[i] + be[i] * std::exp(-ep[i] * p.T));
     }
-    c.range[0] = v[12];
-    c.range[1] = v[13];
+    c.range[0] = rg[0];
+    c.range[1] = rg[1];
     return true;
 }
 

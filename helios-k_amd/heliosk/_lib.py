@@ -10,6 +10,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libheliosk.so")
 
 HK_OK = 0
+HK_ERR_ARG, HK_ERR_HIP, HK_ERR_WORKSPACE, HK_ERR_UNSUPPORTED = 1, 2, 3, 4
 HK_FORM_HITRAN, HK_FORM_EXOMOL, HK_FORM_SUPERLINE = 0, 1, 2
 HK_VOIGT_FP64 = 0
 HK_VOIGT_FP32C = 1
@@ -56,7 +57,7 @@ class LineState(ctypes.Structure):
 
 
 class Tune(ctypes.Structure):
-    _fields_ = [("tile_points", c_int), ("exact_pairs", c_int)]
+    _fields_ = [("tile_points", c_int), ("exact_pairs", c_int), ("stage_lines", c_int)]
 
 
 class Spectrum(ctypes.Structure):
@@ -66,7 +67,7 @@ class Spectrum(ctypes.Structure):
 
 class VoigtBatch(ctypes.Structure):
     _fields_ = [("workspace", c_vp), ("nspec", c_int), ("mode", c_int), ("tile_points", c_int),
-                ("pad", c_int), ("items_main", c_i64), ("items_core", c_i64),
+                ("stage_lines", c_int), ("items_main", c_i64), ("items_core", c_i64),
                 ("off_args", c_size), ("off_cargs", c_size), ("off_items", c_size),
                 ("off_citems", c_size), ("off_counter", c_size)]
 
@@ -116,6 +117,8 @@ def lib():
         L.hk_voigt_work_count.argtypes = [P(LineState), P(Grid), c_int, P(Tune), c_vp, c_vp]
         L.hk_voigt_autotune.argtypes = [P(LineState), P(Grid), c_int, c_vp, c_vp, c_size, P(Tune),
                                         ctypes.POINTER(ctypes.c_float), c_vp]
+        L.hk_voigt_autotune_reset.restype = None
+        L.hk_voigt_autotune_reset.argtypes = []
         L.hk_voigt_batched_workspace_bytes.restype = c_size
         L.hk_voigt_batched_workspace_bytes.argtypes = [c_int, P(Spectrum), P(Tune)]
         L.hk_voigt_batched.argtypes = [c_int, P(Spectrum), c_int, P(Tune), c_vp, c_size, c_vp]
@@ -127,6 +130,10 @@ def lib():
         L.hk_ktable_prepare.argtypes = [P(Grid), c_d, c_int, c_int, c_vp, c_size]
         L.hk_ktable.argtypes = [c_vp, c_int, P(Grid), c_d, c_int, c_int, c_d, c_d, c_vp, c_vp,
                                 c_vp, c_vp, c_size, c_vp]
+        L.hk_ktable_backward_workspace_bytes.restype = c_size
+        L.hk_ktable_backward_workspace_bytes.argtypes = [c_i64, c_int, c_int, c_int]
+        L.hk_ktable_backward.argtypes = [c_vp, c_vp, c_vp, c_int, P(Grid), c_d, c_int, c_int, c_vp,
+                                         c_vp, c_vp, c_size, c_vp]
         L.hk_ktable_reconstruct.argtypes = [c_vp, c_vp, c_int, c_int, c_int, c_int, c_d, c_d,
                                             c_vp, c_vp, c_vp]
         L.hk_ktable_plan_host.argtypes = [c_int, c_int, P_d]
@@ -151,11 +158,13 @@ EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state
             "hk_lines_sort_workspace_bytes", "hk_lines_sort", "hk_means_workspace_bytes",
             "hk_means", "hk_transmission", "hk_profile_workspace_bytes", "hk_profile_accumulate",
             "hk_means_x", "hk_sort_bins_workspace_bytes", "hk_sort_bins",
-            "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate", "hk_voigt_pair_count",
+            "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate",
+"hk_voigt_pair_count",
             "hk_voigt_work_count",
-            "hk_voigt_autotune",
+            "hk_voigt_autotune", "hk_voigt_autotune_reset",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",
             "hk_voigt_batched_launch", "hk_ktable_workspace_bytes",
             "hk_ktable_prepare", "hk_ktable", "hk_ktable_reconstruct", "hk_ktable_plan_host",
+            "hk_ktable_backward_workspace_bytes", "hk_ktable_backward",
             "hk_voigt_adjoint_workspace_bytes", "hk_voigt_adjoint", "hk_bin_segments",
             "hk_synth_lines"]

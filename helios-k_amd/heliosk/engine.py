@@ -21,7 +21,8 @@ NA = 6.0221412927e23          # define.h:42
 
 __all__ = ["HK_FORM_HITRAN", "HK_FORM_EXOMOL", "HK_FORM_SUPERLINE", "Species", "sort_lines",
            "State", "make_grid", "line_params", "voigt", "voigt_batched", "VoigtBatch",
-           "autotune", "ktable", "VoigtOpacity", "opacity", "binned_log_mean",
+           "autotune", "ktable", "KTable", "KTableFit", "ktable_fit", "VoigtOpacity", "opacity",
+           "binned_log_mean",
            "ktable_reconstruct", "adjoint", "synth_lines", "means", "means_x", "sort_bins", "transmission", "profile", "HK_VOIGT_FP64", "HK_VOIGT_FP32C",
            "HK_VOIGT_COMPAT"]
 
@@ -35,6 +36,18 @@ def _dev():
 def _stream(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def _keep(stream, *tensors):
+    """Tensors that work queued on `stream` reads or writes: when that stream
+    is not torch's current stream, the caching allocator is told to wait for
+    it before handing their memory to new work (a transient workspace freed
+    on return would otherwise be reused while the kernels still run)."""
+    if stream is None or stream == torch.cuda.current_stream():
+        return
+    for t in tensors:
+        if t is not None:
+            t.record_stream(stream)
 
 
 def _ptr(t):
@@ -145,6 +158,7 @@ def sort_lines(nu, S, EL, vy, n, A=None, delta=None, iso=None, iso_mass=(18.0 / 
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     check(lib().hk_lines_sort(ctypes.byref(Lin), ctypes.byref(Lout), _ptr(ws), nbytes,
                               _stream(stream)))
+    _keep(stream, ws, raw_iso, out_iso, *raw.values(), *out.values())
     return sp
 
 
@@ -186,16 +200,16 @@ def line_params(species, T, P, Q=None, cut_mode=0, cut=0.0, state=None, stream=N
     return st
 
 
-def _tune(tile_points=0, exact_pairs=False):
-    return Tune(int(tile_points), int(bool(exact_pairs)))
+def _tune(tile_points=0, exact_pairs=False, stage_lines=0):
+    return Tune(int(tile_points), int(bool(exact_pairs)), int(stage_lines))
 
 
 def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, accumulate=False,
-          stream=None, workspace=None, exact_pairs=False):
+          stream=None, workspace=None, exact_pairs=False, stage_lines=0):
     """K_j = kmin + sum_i S_i Re w / sqrt(pi) on the grid (or K += sum).
     exact_pairs: FP32C evaluates every pair instead of interpolating the far
-    field of distant lines."""
-    tune = _tune(tile_points, exact_pairs)
+    field of distant lines; stage_lines: lines per LDS stage (256 or 512)."""
+    tune = _tune(tile_points, exact_pairs, stage_lines)
     if K is None:
         K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
     nbytes = lib().hk_voigt_workspace_bytes(ctypes.byref(grid), ctypes.byref(tune))
@@ -203,6 +217,7 @@ def voigt(state, grid, kmin=0.0, mode=HK_VOIGT_FP32C, tile_points=0, K=None, acc
     check(lib().hk_voigt_accumulate(ctypes.byref(state.st), ctypes.byref(grid), float(kmin),
                                     int(bool(accumulate)), int(mode), ctypes.byref(tune),
                                     _ptr(K), _ptr(ws), ws.numel(), _stream(stream)))
+    _keep(stream, K, ws)
     return K
 
 
@@ -225,6 +240,7 @@ def profile(state, grid, profile, remove_plinth=False, kmin=0.0, voigt_mode=HK_V
                                       int(bool(accumulate)), int(profile), int(bool(remove_plinth)),
                                       int(voigt_mode), None if c is None else ctypes.byref(c),
                                       _ptr(K), _ptr(ws), nbytes, _stream(stream)))
+    _keep(stream, K, ws)
     return K
 
 
@@ -254,26 +270,21 @@ def _spectra(items):
     return arr
 
 
-_TUNED = {}
-
-
-def autotune(state, grid, mode=HK_VOIGT_FP32C, stream=None, cache_key=None):
-    """Fastest main-pass tile size for this input (hk_voigt_autotune), cached
-    per cache_key when one is given.  Returns (tile_points, {size: ms})."""
-    if cache_key is not None and (cache_key, mode) in _TUNED:
-        return _TUNED[(cache_key, mode)]
+def autotune(state, grid, mode=HK_VOIGT_FP32C, stream=None, exact_pairs=False):
+    """Fastest (tile width, lines per LDS stage) of the main pass for this
+    input (hk_voigt_autotune; the library keeps the choice per problem
+    signature for the process).  Returns (tile_points, stage_lines,
+    {(tile, stage): ms}) -- ms 0 for a cached choice."""
     K = torch.empty(grid.nx, dtype=torch.float64, device=_dev())
     tune = _tune(0)
     nbytes = lib().hk_voigt_workspace_bytes(ctypes.byref(grid), ctypes.byref(tune))
     ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
-    best = Tune(0, 0)
-    ms = (ctypes.c_float * 4)()
+    best = _tune(0, exact_pairs)
+    ms = (ctypes.c_float * 8)()
     check(lib().hk_voigt_autotune(ctypes.byref(state.st), ctypes.byref(grid), int(mode), _ptr(K),
                                   _ptr(ws), nbytes, ctypes.byref(best), ms, _stream(stream)))
-    out = (best.tile_points, dict(zip((512, 1024, 2048, 4096), list(ms))))
-    if cache_key is not None:
-        _TUNED[(cache_key, mode)] = out
-    return out
+    keys = [(t, sl) for sl in (256, 512) for t in (512, 1024, 2048, 4096)]
+    return best.tile_points, best.stage_lines, dict(zip(keys, list(ms)))
 
 
 def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
@@ -286,17 +297,19 @@ def voigt_batched(items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
     ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
     check(lib().hk_voigt_batched(n, arr, int(mode), ctypes.byref(tune), _ptr(ws), nbytes,
                                  _stream(stream)))
+    _keep(stream, ws)
 
 
 class VoigtBatch:
     """A prepared batch of spectra: launch() issues kernels only, so it can be
     captured in a torch.cuda.CUDAGraph (a hipGraph) and replayed."""
 
-    def __init__(self, items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None):
+    def __init__(self, items, mode=HK_VOIGT_FP32C, tile_points=0, stream=None,
+                 exact_pairs=False, stage_lines=0):
         self.items = list(items)            # keeps the states and K tensors alive
         n = len(self.items)
         arr = _spectra(self.items)
-        tune = _tune(tile_points)
+        tune = _tune(tile_points, exact_pairs, stage_lines)
         nbytes = lib().hk_voigt_batched_workspace_bytes(n, arr, ctypes.byref(tune))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
         self.batch = _lib.VoigtBatch()
@@ -309,11 +322,16 @@ class VoigtBatch:
 
 
 class KTable:
-    """Prepared k-table workspace for one (grid, numax, nbins, nC, nspec)."""
+    """Prepared k-table workspace for one (grid, numax, nbins, nC, nspec);
+    backward=True sizes it for hk_ktable_backward as well."""
 
-    def __init__(self, grid, numax, nbins, nC, nspec=1):
+    def __init__(self, grid, numax, nbins, nC, nspec=1, backward=False):
         self.grid, self.numax, self.nbins, self.nC, self.nspec = grid, float(numax), nbins, nC, nspec
-        nbytes = lib().hk_ktable_workspace_bytes(grid.nx, nbins, nC, nspec)
+        size = lib().hk_ktable_backward_workspace_bytes if backward else lib().hk_ktable_workspace_bytes
+        nbytes = size(grid.nx, nbins, nC, nspec)
+        if nbytes == 0:
+            raise _lib.HKError("k-table workspace size: unsupported (nx %d, nbins %d, nC %d)"
+                               % (grid.nx, nbins, nC))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device=_dev())
         check(lib().hk_ktable_prepare(ctypes.byref(grid), self.numax, nbins, nC, _ptr(self.ws),
                                       nbytes))
@@ -329,9 +347,50 @@ class KTable:
         check(lib().hk_ktable(_ptr(K), nspec, ctypes.byref(self.grid), self.numax, self.nbins,
                               self.nC, float(kmin), float(unit_scale), _ptr(coeff), _ptr(nxmin),
                               _ptr(Ks), _ptr(self.ws), self.ws.numel(), _stream(stream)))
+        _keep(stream, coeff, nxmin, Ks, self.ws)
         coeff = coeff.view(nspec, self.nbins, self.nC)
         nxmin = nxmin.view(nspec, self.nbins)
         return (coeff, nxmin, Ks) if want_sorted else (coeff, nxmin)
+
+
+    def backward(self, K, Ksorted, nxmin, gcoeff, stream=None):
+        """dL/dK (shaped like K) from dL/dc (hk_ktable_backward): K, Ksorted
+        and nxmin are the input and outputs of the forward call."""
+        nspec = K.numel() // self.grid.nx
+        gc = gcoeff.to(torch.float64).contiguous()
+        Kc, Ksc, nxc = K.contiguous(), Ksorted.contiguous(), nxmin.contiguous()
+        gK = torch.empty_like(Kc)
+        check(lib().hk_ktable_backward(_ptr(Kc), _ptr(Ksc), _ptr(nxc), nspec,
+                                       ctypes.byref(self.grid), self.numax, self.nbins, self.nC,
+                                       _ptr(gc), _ptr(gK), _ptr(self.ws), self.ws.numel(),
+                                       _stream(stream)))
+        _keep(stream, gc, Kc, Ksc, nxc, gK, self.ws)
+        return gK.view_as(K)
+
+
+class KTableFit(torch.autograd.Function):
+    """k-table coefficients (nspec, nbins, nC) of K as a differentiable
+    function; backward runs hk_ktable_backward (through the fit, ln and the
+    rescale; Nxmin and the per-bin order held fixed).  kt: a KTable made with
+    backward=True.  Returns (coeff, nxmin); nxmin is not differentiable."""
+
+    @staticmethod
+    def forward(ctx, K, kt, kmin=0.0, unit_scale=1.0):
+        coeff, nxmin, Ks = kt(K.detach(), kmin, unit_scale, want_sorted=True)
+        ctx.save_for_backward(K.detach(), Ks, nxmin)
+        ctx.kt = kt
+        ctx.mark_non_differentiable(nxmin)
+        return coeff, nxmin
+
+    @staticmethod
+    def backward(ctx, gcoeff, gnxmin):
+        K, Ks, nxmin = ctx.saved_tensors
+        return ctx.kt.backward(K, Ks, nxmin, gcoeff), None, None, None
+
+
+def ktable_fit(K, kt, kmin=0.0, unit_scale=1.0):
+    """Differentiable k-table coefficients, see KTableFit."""
+    return KTableFit.apply(K, kt, kmin, unit_scale)
 
 
 def ktable(K, grid, numax, nbins, nC, kmin=0.0, unit_scale=1.0, want_sorted=False):
@@ -343,9 +402,11 @@ def ktable_reconstruct(coeff, nxmin, Nxb, kmin=0.0, unit_scale=1.0, stream=None)
     nspec, nbins, nC = coeff.shape
     Ky = torch.empty(nspec * nbins * Nxb, dtype=torch.float64, device=coeff.device)
     scratch = torch.empty_like(Ky)
-    check(lib().hk_ktable_reconstruct(_ptr(coeff.contiguous()), _ptr(nxmin.contiguous()), nspec,
-                                      nbins, Nxb, nC, float(kmin), float(unit_scale), _ptr(Ky),
-                                      _ptr(scratch), _stream(stream)))
+    cc, nc = coeff.contiguous(), nxmin.contiguous()
+    check(lib().hk_ktable_reconstruct(_ptr(cc), _ptr(nc), nspec, nbins, Nxb, nC, float(kmin),
+                                      float(unit_scale), _ptr(Ky), _ptr(scratch),
+                                      _stream(stream)))
+    _keep(stream, cc, nc, Ky, scratch)
     return Ky
 
 
@@ -367,9 +428,11 @@ def adjoint(species, T, P, gK, grid, Q=None, cut_mode=0, cut=0.0, stream=None):
     gTP = torch.empty(2, dtype=torch.float64, device=dev)
     nbytes = lib().hk_voigt_adjoint_workspace_bytes(species.nlines)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    gKc = gK.contiguous()
     check(lib().hk_voigt_adjoint(ctypes.byref(L), ctypes.byref(tp), ctypes.byref(c),
-                                 ctypes.byref(grid), _ptr(gK.contiguous()), _ptr(gS), _ptr(gv),
+                                 ctypes.byref(grid), _ptr(gKc), _ptr(gS), _ptr(gv),
                                  _ptr(gTP), _ptr(ws), nbytes, _stream(stream)))
+    _keep(stream, gKc, gS, gv, gTP, ws)
     return gTP, gS, gv
 
 
@@ -422,6 +485,7 @@ def means(K, grid, T, stream=None):
     ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
     check(lib().hk_means(_ptr(K), ctypes.byref(grid), float(T), _ptr(sums), _ptr(ws), nbytes,
                          _stream(stream)))
+    _keep(stream, K, sums, ws)
     return sums
 
 
@@ -433,6 +497,7 @@ def means_x(K, x, T, stream=None):
     ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
     check(lib().hk_means_x(_ptr(K), _ptr(x), K.numel(), float(T), _ptr(sums), _ptr(ws), nbytes,
                            _stream(stream)))
+    _keep(stream, K, x, sums, ws)
     return sums
 
 
@@ -446,6 +511,7 @@ def sort_bins(K, seg, stream=None):
     ws = torch.empty(nbytes, dtype=torch.uint8, device=K.device)
     check(lib().hk_sort_bins(_ptr(K), K.numel(), seg.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                              nseg, _ptr(out), _ptr(ws), nbytes, _stream(stream)))
+    _keep(stream, K, out, ws)
     return out
 
 
@@ -458,6 +524,7 @@ def transmission(Ks, nbins, Nxb, nTr, dTr, kmin=0.0, nxmin=None, stream=None):
         nx = torch.as_tensor(nxmin, dtype=torch.int32).to(Ks.device).contiguous()
     check(lib().hk_transmission(_ptr(Ks), int(nbins), int(Nxb), None if nx is None else _ptr(nx),
                                 float(kmin), int(nTr), float(dTr), _ptr(Tr), _stream(stream)))
+    _keep(stream, Ks, Tr, nx)
     return Tr.view(nbins, nTr)
 
 

@@ -141,10 +141,15 @@ typedef struct {
 
 /* Launch tuning; zero fields select the defaults. */
 typedef struct {
-    int tile_points;    /* grid points per workgroup tile */
+    int tile_points;    /* grid points per workgroup tile: 512, 1024
+                           (default), 2048 or 4096 */
     int exact_pairs;    /* HK_VOIGT_FP32C: 1 evaluates every (line, point)
                            pair; 0 (default) sums lines far from a tile
                            through its Chebyshev far field */
+    int stage_lines;    /* lines staged through LDS per step of the main
+                           pass: 256 (default) or 512 (the reference's fixed
+                           line blocks def_nlA/B/C, define.h:56-58); results
+                           do not depend on it */
 } hk_tune;
 
 const char *hk_last_error(void);
@@ -209,16 +214,22 @@ int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
 
 /* Autotuner (replaces the reference's launch self-tuning, heliosk.cu:1363-
  * 1401): times hk_voigt_accumulate on this input for each tile size in
- * {512, 1024, 2048, 4096} points (one untimed and one timed launch each,
- * device events) and writes the fastest into *best; ms[4] (may be NULL)
- * receives the timings.  K is overwritten; the workspace must be sized
+ * {512, 1024, 2048, 4096} points and each stage of {256, 512} lines (one
+ * untimed and one timed launch each, device events) and writes the fastest
+ * pair into *best; ms[8] (may be NULL) receives the timings, tile-major
+ * within each stage size (ms[4 * (stage == 512) + tile index]).  K is overwritten; the workspace must be sized
  * for the smallest tile (hk_voigt_workspace_bytes covers every size).
  * Waits for the stream.  Results of different tile sizes agree to
- * rounding, not bit for bit.  best->exact_pairs is read: the sizes are
- * timed with that setting. */
+ * rounding, not bit for bit, so the choice is cached for the process per
+ * problem signature (form, exact_pairs, grid, line count, and the lines'
+ * cut reach and median y, each to 3 significant digits): a repeated tuning
+ * of the same problem returns the first choice without timing (ms = 0), and
+ * reruns give the same bits.  best->exact_pairs is read: the sizes are
+ * timed with that setting.  hk_voigt_autotune_reset empties the cache. */
 int hk_voigt_autotune(const hk_line_state *st, const hk_grid *grid, int mode,
                       double *K, void *workspace, size_t workspace_bytes,
                       hk_tune *best, float *ms, hk_stream_t stream);
+void hk_voigt_autotune_reset(void);
 
 /* Many spectra in one persistent launch (work queue over spectrum tiles). */
 typedef struct {
@@ -242,7 +253,7 @@ int hk_voigt_batched(int nspec, const hk_spectrum *spec, int mode,
  * spectrum from the current contents of the line states. */
 typedef struct {
     void *workspace;
-    int nspec, mode, tile_points, pad;
+    int nspec, mode, tile_points, stage_lines;
     int64_t items_main, items_core;
     size_t off_args, off_cargs, off_items, off_citems, off_counter;
 } hk_voigt_batch;
@@ -266,6 +277,23 @@ int hk_ktable(const double *K, int nspec, const hk_grid *grid, double numax,
               int nbins, int nC, double kmin, double unit_scale,
               double *coeff, int32_t *nxmin, double *Ksorted,
               void *workspace, size_t workspace_bytes, hk_stream_t stream);
+/* Gradient of the k-table coefficients with respect to the spectra (no
+ * reference counterpart: the backward of the sequence hk_ktable replaces,
+ * heliosk.cu:2046-2193 and resample.h:243-481).  gcoeff = dL/dc in
+ * hk_ktable's coeff layout; writes gK = dL/dK (nspec x nx).  K is the input
+ * of the hk_ktable call, Ksorted and nxmin its outputs (the sorted bins are
+ * required).  The derivative runs through the least-squares fit, ln and the
+ * rescale (linear between sorted values); Nxmin and the sort order are
+ * piecewise constant in K and held fixed; values equal within a bin share
+ * the mean gradient of their ranks.  The workspace, of
+ * hk_ktable_backward_workspace_bytes, must have been hk_ktable_prepare'd
+ * (it may also serve the forward call).  Bins must be the fixed Nxb chunks
+ * of the grid (HK_ERR_UNSUPPORTED otherwise). */
+size_t hk_ktable_backward_workspace_bytes(int64_t nx, int nbins, int nC, int nspec);
+int hk_ktable_backward(const double *K, const double *Ksorted, const int32_t *nxmin,
+                       int nspec, const hk_grid *grid, double numax, int nbins, int nC,
+                       const double *gcoeff, double *gK, void *workspace,
+                       size_t workspace_bytes, hk_stream_t stream);
 /* K(y) from the coefficients: exp(Chebyshev sum) on y = k/Nxb, mapped back
  * over [Nxmin, Nxb) (expfx_kernel + rescale f=-1).  scratch: nspec*nbins*Nxb
  * doubles of device memory. */

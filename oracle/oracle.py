@@ -25,46 +25,81 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _bind(L):
+    c_d, c_i64, c_int = ctypes.c_double, ctypes.c_int64, ctypes.c_int
+    L.hko_erfcx.restype = c_d
+    L.hko_erfcx.argtypes = [c_d]
+    L.hko_faddeeva_batch.argtypes = [c_i64, _D, _D, _D, _D]
+    L.hko_w916.argtypes = [c_d, c_d, c_d, _D, _D]
+    L.hko_wcf.argtypes = [c_d, c_d, c_int, _D, _D]
+    L.hko_line_params.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
+                                  + [c_d, c_d, c_int, c_d, c_int] + [_D] * 5)
+    L.hko_window.argtypes = [c_d, c_d, c_d, c_d, c_i64, _I64, _I64]
+    L.hko_voigt_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d,
+                                                        c_i64, _I64, _D, c_int])
+    L.hko_voigt_compat_points.argtypes = L.hko_voigt_points.argtypes
+    L.hko_compat_profile.restype = ctypes.c_float
+    L.hko_compat_profile.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.hko_voigt_grad.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
+                                 + [c_d, c_d, c_int, c_d, c_int, c_d, c_d, c_i64, _D,
+                                    _D, _D, _D, c_int])
+    L.hko_bin_segments.argtypes = [c_d, c_d, c_d, c_i64, c_int, _I64]
+    L.hko_sort_segments.argtypes = [_D, c_int, _I64]
+    L.hko_qr.argtypes = [c_int, c_int, _D, _D, _D]
+    L.hko_ktable.argtypes = [_D, c_i64, c_int, c_int, c_int, c_d, c_d, _D, _INT]
+    L.hko_lsq.argtypes = [c_int, c_int, _D, _D]
+    L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
+    L.hko_means.argtypes = [_D, c_d, c_d, c_i64, c_d, _D]
+    L.hko_profile_value.restype = c_d
+    L.hko_profile_value.argtypes = [c_int, c_d, c_d, c_d, c_d]
+    L.hko_profile_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d, c_int,
+                                                          c_int, _D, _D, c_int])
+    L.hko_chi.restype = c_d
+    L.hko_chi.argtypes = [_D, c_d]
+    L.hko_means_x.argtypes = [_D, _D, c_i64, c_d, _D]
+    L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
+    return L
+
+
 def lib():
+    """The checker: built with -march=x86-64-v3 -ffp-contract=off."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        c_d, c_i64, c_int = ctypes.c_double, ctypes.c_int64, ctypes.c_int
-        L.hko_erfcx.restype = c_d
-        L.hko_erfcx.argtypes = [c_d]
-        L.hko_faddeeva_batch.argtypes = [c_i64, _D, _D, _D, _D]
-        L.hko_w916.argtypes = [c_d, c_d, c_d, _D, _D]
-        L.hko_wcf.argtypes = [c_d, c_d, c_int, _D, _D]
-        L.hko_line_params.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
-                                      + [c_d, c_d, c_int, c_d, c_int] + [_D] * 5)
-        L.hko_window.argtypes = [c_d, c_d, c_d, c_d, c_i64, _I64, _I64]
-        L.hko_voigt_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d,
-                                                            c_i64, _I64, _D, c_int])
-        L.hko_voigt_compat_points.argtypes = L.hko_voigt_points.argtypes
-        L.hko_compat_profile.restype = ctypes.c_float
-        L.hko_compat_profile.argtypes = [ctypes.c_float, ctypes.c_float]
-        L.hko_voigt_grad.argtypes = ([c_i64] + [_D] * 7 + [_I32] + [_D] * 4
-                                     + [c_d, c_d, c_int, c_d, c_int, c_d, c_d, c_i64, _D,
-                                        _D, _D, _D, c_int])
-        L.hko_bin_segments.argtypes = [c_d, c_d, c_d, c_i64, c_int, _I64]
-        L.hko_sort_segments.argtypes = [_D, c_int, _I64]
-        L.hko_qr.argtypes = [c_int, c_int, _D, _D, _D]
-        L.hko_ktable.argtypes = [_D, c_i64, c_int, c_int, c_int, c_d, c_d, _D, _INT]
-        L.hko_lsq.argtypes = [c_int, c_int, _D, _D]
-        L.hko_reconstruct.argtypes = [_D, _INT, c_int, c_int, c_int, c_d, c_d, _D]
-        L.hko_means.argtypes = [_D, c_d, c_d, c_i64, c_d, _D]
-        L.hko_profile_value.restype = c_d
-        L.hko_profile_value.argtypes = [c_int, c_d, c_d, c_d, c_d]
-        L.hko_profile_points.argtypes = ([c_i64] + [_D] * 5 + [c_d, c_d, c_d, c_i64, c_d, c_int,
-                                                              c_int, _D, _D, c_int])
-        L.hko_chi.restype = c_d
-        L.hko_chi.argtypes = [_D, c_d]
-        L.hko_means_x.argtypes = [_D, _D, c_i64, c_d, _D]
-        L.hko_transmission.argtypes = [_D, c_int, c_int, _I32, c_d, c_int, c_d, _D]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+_native = None
+
+
+def lib_native():
+    """The same source built for the CPU this runs on (-O3 -march=native
+    -fopenmp, the compiler's default contraction), compiled on first use
+    into a directory keyed by the host's CPU model: the timed CPU baseline
+    only.  Returns (library, build description)."""
+    global _native
+    if _native is None:
+        import hashlib
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith(("model name", "flags")):
+                        model += line
+                        if line.startswith("flags"):
+                            break
+        except OSError:
+            pass
+        tag = hashlib.sha1(model.encode()).hexdigest()[:12]
+        out = os.path.join(_HERE, "_build", "native-" + tag)
+        path = os.path.join(out, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", _HERE, "native", "NATIVE_OUT=" + out], check=True)
+        _native = (_bind(ctypes.CDLL(path)),
+                   "gcc -O3 -march=native -fopenmp (built on this host, %s)" % tag)
+    return _native
 
 
 def _p(a, t=_D):
@@ -106,18 +141,18 @@ def _iso_arrays(lines):
     return iso, tabs
 
 
-def line_params(lines, T, P, cut_mode, cut, form=0):
+def line_params(lines, T, P, cut_mode, cut, form=0, L=None):
     """Line state at (T, P[atm]): dict nu, S, ialphaD, y, hw.  form: 0 HITRAN
     (S2_kernel), 1 ExoMol family (Sf_kernel), 2 super-lines (SfSuper_kernel)."""
     nl = len(lines["nu"])
     cols = [_f64(lines[k]) for k in ("nu", "S", "EL", "A", "delta", "vy", "n")]
     iso, tabs = _iso_arrays(lines)
     out = {k: np.empty(nl) for k in ("nu", "S", "ialphaD", "y", "hw")}
-    lib().hko_line_params(nl, *[_p(c) for c in cols],
-                          None if iso is None else _p(iso, _I32),
-                          *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
-                          float(cut), int(form),
-                          *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
+    (L or lib()).hko_line_params(nl, *[_p(c) for c in cols],
+                                 None if iso is None else _p(iso, _I32),
+                                 *[_p(t) for t in tabs], float(T), float(P), int(cut_mode),
+                                 float(cut), int(form),
+                                 *[_p(out[k]) for k in ("nu", "S", "ialphaD", "y", "hw")])
     return out
 
 
@@ -156,7 +191,7 @@ def window(nu, hw, numin, dnu, Nx):
 
 
 def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
-                 nthreads=0):
+                 nthreads=0, L=None):
     """K at grid points jidx (all points if None).  The C routine searches
     the lines by centre, so a state whose (pressure-shifted) centres are not
     ascending is put in order first."""
@@ -174,13 +209,13 @@ def voigt_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, search_radius=None,
         jidx = np.ascontiguousarray(jidx, dtype=np.int64); npts = jidx.size
         jp = _p(jidx, _I64)
     K = np.empty(npts)
-    lib().hko_voigt_points(nl, *[_p(c) for c in cols], float(search_radius),
-                           float(numin), float(dnu), int(Nx), float(kmin), npts, jp,
-                           _p(K), int(nthreads))
+    (L or lib()).hko_voigt_points(nl, *[_p(c) for c in cols], float(search_radius),
+                                  float(numin), float(dnu), int(Nx), float(kmin), npts, jp,
+                                  _p(K), int(nthreads))
     return K
 
 
-def voigt_compat_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, nthreads=0):
+def voigt_compat_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, nthreads=0, L=None):
     """K at grid points jidx (all if None) in the reference's own FP32
     arithmetic (hko_voigt_compat_points), lines in centre order."""
     cols = [_f64(state[k]) for k in ("nu", "S", "ialphaD", "y", "hw")]
@@ -196,8 +231,9 @@ def voigt_compat_points(state, numin, dnu, Nx, kmin=0.0, jidx=None, nthreads=0):
         jidx = np.ascontiguousarray(jidx, dtype=np.int64); npts = jidx.size
         jp = _p(jidx, _I64)
     K = np.empty(npts)
-    lib().hko_voigt_compat_points(nl, *[_p(c) for c in cols], R, float(numin), float(dnu),
-                                  int(Nx), float(kmin), npts, jp, _p(K), int(nthreads))
+    (L or lib()).hko_voigt_compat_points(nl, *[_p(c) for c in cols], R, float(numin),
+                                         float(dnu), int(Nx), float(kmin), npts, jp, _p(K),
+                                         int(nthreads))
     return K
 
 
@@ -231,10 +267,10 @@ def bin_segments(numin, numax, dnu, Nx, nbins):
     return seg
 
 
-def sort_segments(K, seg):
+def sort_segments(K, seg, L=None):
     K = _f64(K).copy()
     seg = np.ascontiguousarray(seg, dtype=np.int64)
-    lib().hko_sort_segments(_p(K), int(seg.size - 1), _p(seg, _I64))
+    (L or lib()).hko_sort_segments(_p(K), int(seg.size - 1), _p(seg, _I64))
     return K
 
 
@@ -244,12 +280,12 @@ def qr(Nxb, nC):
     return V.reshape(nC, Nxb), C, D
 
 
-def ktable(Ksorted, Nxb, nbins, nC, kmin, unit_scale=1.0):
+def ktable(Ksorted, Nxb, nbins, nC, kmin, unit_scale=1.0, L=None):
     """Coefficients [nbins, nC] and Nxmin [nbins] from a per-bin sorted K."""
     K = _f64(Ksorted).copy()
     coeff = np.empty(nbins * nC); nxmin = np.empty(nbins, dtype=np.intc)
-    lib().hko_ktable(_p(K), K.size, int(Nxb), int(nbins), int(nC), float(kmin),
-                     float(unit_scale), _p(coeff), _p(nxmin, _INT))
+    (L or lib()).hko_ktable(_p(K), K.size, int(Nxb), int(nbins), int(nC), float(kmin),
+                            float(unit_scale), _p(coeff), _p(nxmin, _INT))
     return coeff.reshape(nbins, nC), nxmin.astype(np.int64)
 
 

@@ -577,3 +577,142 @@ def test_driver_sublorentzian(tmp_path):
     e = rel_err(blk[:, 1], Ko)
     print("driver sub-Lorentzian vs oracle %.3e" % e)
     assert e <= 1e-11
+
+
+def _run_any(cwd):
+    """The driver's stdout, errors allowed (the reference exits with status 0
+    after printing Error: ...)."""
+    r = subprocess.run([DRIVER], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    return r.stdout
+
+
+def test_driver_per_bin_cbin_files_and_append(tmp_path):
+    """doResampling = 2 writes one _cbin%04d.dat per bin (heliosk.cu:2166-2187):
+    for each pressure the bin's row -- kmin, ystart, nC coefficients, T, P --
+    ending with a newline except in the last bin's file; the rows equal
+    doResampling = 1's.  ReplaceFiles = 0 appends a second run's rows,
+    ReplaceFiles = 1 starts the files again."""
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 2000, 31, 995.0, 1015.0)
+    (tmp_path / "P.txt").write_text("0.5\n2.0\n")
+    common = dict(name="r", T=1400.0, P=1.0, PFile="P.txt", Species_Name="01_tst", numin=NUMIN,
+                  numax=NUMAX, dnu=DNU, cut=5.0, nC=NC, doStoreFullK=0, nbins=NBINS)
+    F.write_param_dat(str(tmp_path / "param.dat"), doResampling=1, **common)
+    run_driver(str(tmp_path))
+    rows = F.read_cbin(str(tmp_path / "Out_r_cbin.dat"), NC)        # pressure-major rows
+    F.write_param_dat(str(tmp_path / "param.dat"), doResampling=2, ReplaceFiles=1, **common)
+    run_driver(str(tmp_path))
+
+    def bin_tokens():
+        out = []
+        for b in range(NBINS):
+            with open(tmp_path / ("Out_r_cbin%04d.dat" % b)) as f:
+                text = f.read()
+            out.append((text, np.array([float(v) for v in text.split()])))
+        return out
+
+    first = bin_tokens()
+    width = 2 + NC + 2
+    for b, (text, tok) in enumerate(first):
+        assert tok.size == 2 * width
+        for iP in range(2):
+            got = tok[iP * width:(iP + 1) * width]
+            ref = np.concatenate([[rows["kmin"][iP * NBINS + b], rows["ystart"][iP * NBINS + b]],
+                                  rows["coeff"][iP * NBINS + b], rows["TP"][iP * NBINS + b]])
+            np.testing.assert_array_equal(got, ref)
+        assert text.count("\n") == (2 if b < NBINS - 1 else 0)
+    # a second run appends (ReplaceFiles = 0) ...
+    F.write_param_dat(str(tmp_path / "param.dat"), doResampling=2, ReplaceFiles=0, **common)
+    run_driver(str(tmp_path))
+    for (t1, k1), (t2, k2) in zip(first, bin_tokens()):
+        assert t2 == t1 + t1
+    # ... and ReplaceFiles = 1 starts over
+    F.write_param_dat(str(tmp_path / "param.dat"), doResampling=2, ReplaceFiles=1, **common)
+    run_driver(str(tmp_path))
+    for (t1, _), (t2, _) in zip(first, bin_tokens()):
+        assert t2 == t1
+
+
+def test_driver_replace_files_zero_appends_cross_sections(tmp_path):
+    """ReplaceFiles = 0 opens Out_<name>.dat / .bin for appending
+    (heliosk.cu:373-379): two runs leave two copies, ReplaceFiles = 1 one."""
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 800, 33, 995.0, 1015.0)
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    for fullk, reader in ((1, lambda: F.read_out_dat(str(tmp_path / "Out_a.dat"))),
+                          (2, lambda: F.read_out_bin(str(tmp_path / "Out_a.bin"), nx))):
+        common = dict(name="a", T=900.0, P=1.0, Species_Name="01_tst", numin=NUMIN, numax=NUMAX,
+                      dnu=DNU, cut=5.0, doStoreFullK=fullk, nbins=NBINS)
+        F.write_param_dat(str(tmp_path / "param.dat"), ReplaceFiles=1, **common)
+        run_driver(str(tmp_path))
+        one = reader()
+        F.write_param_dat(str(tmp_path / "param.dat"), ReplaceFiles=0, **common)
+        run_driver(str(tmp_path))
+        two = reader()
+        assert len(two) == 2 * len(one)
+        for blk in two:
+            np.testing.assert_array_equal(blk, one[0])
+        F.write_param_dat(str(tmp_path / "param.dat"), ReplaceFiles=1, **common)
+        run_driver(str(tmp_path))
+        assert len(reader()) == len(one)
+
+
+def test_driver_fullk_text_reload(tmp_path):
+    """doStoreFullK = -1 reads the cross sections back from
+    <pathToK>Out_<name>.dat (heliosk.cu:1862-1935) and runs only the k-table:
+    %.20g keeps every double, so the k-tables equal the first run's bit for
+    bit, also with two pressures (four columns)."""
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 1500, 35, 995.0, 1015.0)
+    (tmp_path / "P.txt").write_text("0.3\n4.0\n")
+    os.mkdir(tmp_path / "kdir")
+    common = dict(name="k", T=1600.0, P=1.0, PFile="P.txt", Species_Name="01_tst", numin=NUMIN,
+                  numax=NUMAX, dnu=DNU, cut=5.0, nC=NC, doResampling=1, nbins=NBINS, kmin=1e-24)
+    F.write_param_dat(str(tmp_path / "param.dat"), doStoreFullK=1, **common)
+    run_driver(str(tmp_path))
+    first = open(tmp_path / "Out_k_cbin.dat").read()
+    os.rename(tmp_path / "Out_k.dat", tmp_path / "kdir" / "Out_k.dat")
+    os.remove(tmp_path / "Out_k_cbin.dat")
+    F.write_param_dat(str(tmp_path / "param.dat"), doStoreFullK=-1, pathToK="kdir/", **common)
+    out = run_driver(str(tmp_path))
+    assert open(tmp_path / "Out_k_cbin.dat").read() == first
+    # a file with too few rows is refused with the reference's message
+    blk = open(tmp_path / "kdir" / "Out_k.dat").read().splitlines()
+    (tmp_path / "kdir" / "Out_k.dat").write_text("\n".join(blk[:100]) + "\n")
+    out = _run_any(str(tmp_path))
+    assert "Error: Input file too short kdir/Out_k.dat" in out, out
+
+
+@pytest.mark.parametrize("T,msg", [(0.5, "Error: partition function not valid for given temperature. T1: 1, T: 0.5"),
+                                   (3500.0, "Error: partition function not valid for given temperature. T0: 3000, T: 3500"),
+                                   (1.0, None), (2999.5, None)])
+def test_driver_partition_range(tmp_path, T, msg):
+    """readPartition's rules (host.h:9-73): T below the table's first row or
+    beyond its last is refused with the reference's messages; T on the first
+    row and between the last two rows are interpolated (q_at)."""
+    sp = make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 300, 37, 995.0, 1015.0)
+    F.write_param_dat(str(tmp_path / "param.dat"), name="q", T=T, P=1.0, Species_Name="01_tst",
+                      numin=NUMIN, numax=NUMAX, dnu=DNU, cut=5.0, doStoreFullK=1, nbins=NBINS)
+    out = _run_any(str(tmp_path))
+    if msg is not None:
+        assert msg in out, out
+        return
+    assert "Error" not in out, out
+    nx = int((NUMAX - NUMIN) / DNU + 0.5)
+    meanMass = float(np.sum(sp["abund"] * sp["masses"]))
+    Ko = oracle_K([sp], [1.0], meanMass, T, 1.0, 1.0, 1.0, 5.0, NUMIN, DNU, nx, 0.0)
+    assert rel_err(F.read_out_dat(str(tmp_path / "Out_q.dat"))[0][:, 1], Ko) <= 1e-11
+
+
+def test_driver_partition_unreadable(tmp_path):
+    """An empty partition table cannot be read; a table with a text row stops
+    there (the reference's fscanf loop), so T beyond it is not covered."""
+    make_species(tmp_path, "01_tst", ["11"], [18.010565], [0.997317], 300, 38, 995.0, 1015.0)
+    F.write_param_dat(str(tmp_path / "param.dat"), name="q", T=1500.0, P=1.0,
+                      Species_Name="01_tst", numin=NUMIN, numax=NUMAX, dnu=DNU, cut=5.0,
+                      doStoreFullK=1, nbins=NBINS)
+    qfile = "q_01_tst_0.txt"                     # F.species_files' partition table
+    assert os.path.exists(tmp_path / qfile)
+    (tmp_path / qfile).write_text("")
+    assert "Error: partition function can not be read" in _run_any(str(tmp_path))
+    (tmp_path / qfile).write_text("1 2\n100 3\nabc def\n2000 4\n")
+    out = _run_any(str(tmp_path))
+    assert "Error: partition function not valid for given temperature. T0: 100, T: 1500" in out, out

@@ -91,3 +91,81 @@ def test_segment_rules_match_oracle(E):
         g = E.make_grid(numin, numax, dnu)
         np.testing.assert_array_equal(E.bin_segments(g, numax, nbins),
                                       O.bin_segments(numin, numax, dnu, g.nx, nbins))
+
+
+@pytest.mark.parametrize("Nxb,nbins,nC,kmin", [(5000, 3, 40, 0.0), (2000, 2, 100, 1e-9),
+                                               (30000, 2, 100, 0.0), (12, 4, 12, 0.0),
+                                               (50, 3, 1, 0.0), (40000, 1, 2, 1e-9),
+                                               (16384, 2, 33, 0.0)])
+def test_nc_range_vs_oracle(E, Nxb, nbins, nC, kmin):
+    """nC from 1 to the reference's maximum of 100 (def_NmaxSample), bins as
+    short as nC points (the square QR of resample.h) and bins of both sort
+    paths (one LDS sort up to 16384 points, runs + buckets above)."""
+    rng = np.random.default_rng(nC * 7 + Nxb)
+    nx = Nxb * nbins
+    K = np.exp(rng.normal(-25.0, 3.0, nx))
+    if kmin > 0:
+        K[: Nxb // 4] = kmin
+    grid = E.make_grid(0.0, float(nx), 1.0)
+    coeff, nxmin, Ks = E.ktable(_dev(K), grid, float(nx), nbins, nC, kmin=kmin, want_sorted=True)
+    seg = O.bin_segments(0.0, float(nx), 1.0, nx, nbins)
+    so = O.sort_segments(K, seg[:nbins + 1])
+    np.testing.assert_array_equal(Ks.cpu().numpy(), so)
+    co, no = O.ktable(so, Nxb, nbins, nC, kmin)
+    np.testing.assert_array_equal(nxmin[0].cpu().numpy(), no)
+    c = coeff[0].cpu().numpy()
+    scale = np.max(np.abs(co), axis=1, keepdims=True)
+    err = np.max(np.abs(c - co) / scale)
+    print("Nxb %d nC %d: coefficients vs oracle %.2e" % (Nxb, nC, err))
+    assert err <= 1e-9
+
+
+def test_many_spectra_beyond_grid_y_limit(E):
+    """nspec * (nbins + 1) = 66220 segments (more than 65535 workgroups in one
+    grid dimension): one call equals per-spectrum calls bit for bit."""
+    import torch
+    nspec, nbins, Nxb, nC = 220, 300, 20, 4
+    nx = nbins * Nxb
+    rng = np.random.default_rng(5)
+    K = np.exp(rng.normal(-20.0, 2.0, (nspec, nx)))
+    grid = E.make_grid(0.0, float(nx), 1.0)
+    coeff, nxmin, Ks = E.ktable(_dev(K.ravel()), grid, float(nx), nbins, nC, want_sorted=True)
+    kt1 = E.KTable(grid, float(nx), nbins, nC)
+    for s in (0, 1, 117, nspec - 1):
+        c1, n1, k1 = kt1(_dev(K[s]), want_sorted=True)
+        assert torch.equal(coeff[s], c1[0]) and torch.equal(nxmin[s], n1[0])
+        assert torch.equal(Ks.view(nspec, nx)[s], k1)
+
+
+@pytest.mark.parametrize("Nxb,kind", [(300000, "ties"), (1200000, "kmin"), (70000, "ties")])
+def test_long_bins_with_ties(E, Nxb, kind):
+    """Bins longer than 16 LDS-sorted runs (merge passes before the bucket
+    pass) and bins full of equal values (the bucket splitters repeat): the
+    sorted bin equals numpy's sort and the fit equals the oracle's."""
+    rng = np.random.default_rng(Nxb)
+    if kind == "ties":
+        K = rng.integers(1, 40, Nxb).astype(np.float64) * 1e-22
+        kmin = 0.0
+    else:
+        K = np.where(rng.random(Nxb) < 0.8, 1e-12, np.exp(rng.normal(-20.0, 3.0, Nxb)))
+        kmin = 1e-12
+    grid = E.make_grid(0.0, float(Nxb), 1.0)
+    coeff, nxmin, Ks = E.ktable(_dev(K), grid, float(Nxb), 1, 20, kmin=kmin, want_sorted=True)
+    so = np.sort(K)
+    np.testing.assert_array_equal(Ks.cpu().numpy(), so)
+    co, no = O.ktable(so, Nxb, 1, 20, kmin)
+    assert int(nxmin[0, 0]) == int(no[0])
+    c = coeff[0, 0].cpu().numpy()
+    assert np.max(np.abs(c - co[0])) / np.max(np.abs(co[0])) <= 1e-11
+
+
+def test_sort_bins_mixed_lengths(E):
+    """hk_sort_bins on segments of 0, 1, 17, 16384, 16385 and 250000 points in
+    one call (the LDS and the run + bucket paths together)."""
+    lens = [0, 1, 17, 16384, 16385, 250000, 3]
+    seg = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    rng = np.random.default_rng(11)
+    K = np.round(rng.normal(0.0, 1.0, int(seg[-1])), 3)      # signed values with ties
+    Ks = E.sort_bins(_dev(K), seg).cpu().numpy()
+    for b in range(len(lens)):
+        np.testing.assert_array_equal(Ks[seg[b]:seg[b + 1]], np.sort(K[seg[b]:seg[b + 1]]))

@@ -262,7 +262,11 @@ def test_config0_fp64_and_fp32c_subsample(E):
 
 def test_config1_precision_at_scale(E):
     """Workload c1 (1e7 lines, 3e7 points, 2000 K, 10 bar, cut 25): FP32C vs
-    FP64 on the whole grid and both vs the oracle on 2000 random points."""
+    FP64 on the whole grid, both against the oracle on a seeded 1 % random
+    subsample of the grid (300000 points), and the FP32C k-tables (300 bins x
+    20) against the FP64 ones and the oracle's fit of the FP64 spectrum:
+    identical Nxmin (ystart), coefficients within 1e-4."""
+    import torch
     from heliosk import configs
     w = configs.WORKLOADS["c1"]
     arr = configs.species_arrays(w, 0)
@@ -272,16 +276,25 @@ def test_config1_precision_at_scale(E):
     grid = _grid(E, w.numin, w.numax, w.dnu)
     K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64)
     K32 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C)
-    import torch
     tau = 1e-12 * float(K64.max())
     e3264 = float(torch.max(torch.abs(K32 - K64) / torch.clamp(torch.abs(K64), min=tau)))
-    jj = np.sort(np.random.default_rng(1).choice(grid.nx, 2000, replace=False))
+    jj = np.sort(np.random.default_rng(1).choice(grid.nx, grid.nx // 100, replace=False))
     Ko = O.voigt_points(state_numpy(st), grid.numin, grid.dnu, grid.nx, jidx=jj)
     k64 = K64.cpu().numpy()[jj]; k32 = K32.cpu().numpy()[jj]
     e64, e32 = rel_err(k64, Ko), rel_err(k32, Ko)
-    print("c1: FP64 vs oracle %.3e, FP32C vs oracle %.3e, FP32C vs FP64 (full grid) %.3e"
-          % (e64, e32, e3264))
+    print("c1: FP64 vs oracle %.3e, FP32C vs oracle %.3e (%d points), FP32C vs FP64 (full grid) "
+          "%.3e" % (e64, e32, jj.size, e3264))
     assert e64 <= FP64_TOL and e32 <= FP32C_TOL and e3264 <= FP32C_TOL
+    kt = E.KTable(grid, w.numax, w.nbins, w.nC)
+    c32, n32 = kt(K32)
+    c64, n64, Ks64 = kt(K64, want_sorted=True)
+    assert torch.equal(n32, n64)
+    d = float(torch.max(torch.abs(c32 - c64)))
+    co, no = O.ktable(Ks64.cpu().numpy(), grid.nx // w.nbins, w.nbins, w.nC, 0.0)
+    np.testing.assert_array_equal(n64[0].cpu().numpy(), no)
+    do = float(np.max(np.abs(c64[0].cpu().numpy() - co)))
+    print("c1 k-tables: FP32C vs FP64 max |dc| %.3e, FP64 vs oracle %.3e" % (d, do))
+    assert d <= 1e-4 and do <= 1e-9
 
 
 @pytest.mark.parametrize("n", [1, 1000, 300001])
@@ -461,3 +474,66 @@ def test_batched_extreme_tp_vs_oracle(E):
             assert e <= (FP64_TOL if mode == E.HK_VOIGT_FP64 else FP32C_TOL)
     for K32, K64 in zip(res[E.HK_VOIGT_FP32C], res[E.HK_VOIGT_FP64]):
         assert rel_err(K32, K64) <= FP32C_TOL
+
+
+def test_side_stream_keeps_transient_buffers(E):
+    """Calls queued on a stream other than torch's current one keep their
+    transient workspaces until that stream is done: the side stream is held
+    busy by a long sum first, the current stream then takes freed memory
+    right away and zeroes it, and the side-stream results still equal the
+    current-stream results bit for bit."""
+    import torch
+    d = random_lines(3000, seed=77, numin=990.0, numax=1030.0)
+    sp = species_from(d)
+    st = E.line_params(sp, 1500.0, 1.0, Q=d["iso_Q"], cut_mode=0, cut=10.0)
+    grid = _grid(E, 1000.0, 1020.0, 0.0005)
+    big = random_lines(200000, seed=78, numin=0.0, numax=3000.0)
+    stb = E.line_params(species_from(big), 1500.0, 1.0, Q=big["iso_Q"], cut_mode=0, cut=50.0)
+    gbig = _grid(E, 0.0, 3000.0, 0.001)
+    modes = (E.HK_VOIGT_FP64, E.HK_VOIGT_FP32C)
+    refs = {m: E.voigt(st, grid, 0.0, m).cpu() for m in modes}
+    kref = E.ktable(refs[E.HK_VOIGT_FP64].cuda(), grid, 1020.0, 4, 8)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    Kb = E.voigt(stb, gbig, 0.0, E.HK_VOIGT_FP64, stream=side)      # keeps the side stream busy
+    outs = {}
+    for m in modes:
+        outs[m] = E.voigt(st, grid, 0.0, m, stream=side)
+        junk = [torch.zeros(1 << 20, dtype=torch.float64, device="cuda") for _ in range(8)]
+        del junk
+    kt = E.KTable(grid, 1020.0, 4, 8)
+    Kin = refs[E.HK_VOIGT_FP64].cuda()
+    side.wait_stream(torch.cuda.current_stream())
+    c, n = kt(Kin, stream=side)
+    torch.cuda.synchronize()
+    for m in modes:
+        assert torch.equal(outs[m].cpu(), refs[m]), m
+    assert torch.equal(c.cpu(), kref[0].cpu()) and torch.equal(n.cpu(), kref[1].cpu())
+    del Kb
+
+
+def test_stage_lines_do_not_change_results(E):
+    """512 lines per LDS stage against 256, every tile width, both forms,
+    single and batched launches: bit-identical sums (lines are added in the
+    same order), and the autotuner returns a valid (tile, stage) pair that a
+    second tuning of the same problem repeats from its cache."""
+    import torch
+    d = random_lines(40000, seed=48, numin=900.0, numax=1100.0)
+    st = E.line_params(species_from(d), 1700.0, 3.0, Q=d["iso_Q"], cut_mode=0, cut=25.0)
+    grid = _grid(E, 950.0, 1050.0, 0.001)
+    for mode in (E.HK_VOIGT_FP32C, E.HK_VOIGT_FP64):
+        for tp in (512, 1024, 2048, 4096):
+            a = E.voigt(st, grid, 0.0, mode, tile_points=tp, stage_lines=256)
+            b = E.voigt(st, grid, 0.0, mode, tile_points=tp, stage_lines=512)
+            assert torch.equal(a, b), (mode, tp)
+        K = torch.empty(grid.nx, dtype=torch.float64, device="cuda")
+        E.VoigtBatch([(st, grid, 0.0, K)], mode=mode, tile_points=1024, stage_lines=512).launch()
+        torch.cuda.synchronize()
+        assert torch.equal(K, E.voigt(st, grid, 0.0, mode, tile_points=1024))
+    E.lib().hk_voigt_autotune_reset()
+    t1, s1, ms1 = E.autotune(st, grid, E.HK_VOIGT_FP32C)
+    t2, s2, ms2 = E.autotune(st, grid, E.HK_VOIGT_FP32C)
+    assert t1 in (512, 1024, 2048, 4096) and s1 in (256, 512)
+    assert (t2, s2) == (t1, s1) and all(v > 0 for v in ms1.values())
+    assert all(v == 0 for v in ms2.values())
+    print("autotuned: tile %d, stage %d; %s" % (t1, s1, ms1))
