@@ -1344,6 +1344,54 @@ extern "C" int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
 }
 
 // ---------------------------------------------------------------------------
+// No-allocation form: the tile tables in a static device array, one per GPU.
+// The array's name is a host-side handle only; its device address is looked
+// up once per device (hipGetSymbolAddress, with that device current) and that
+// address is what the kernels receive.  The tile tables are rewritten in full
+// by every call, so nothing in the array needs clearing between calls.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[HK_VOIGT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(void **out) {
+    int dev = 0;
+    HK_HIP(hipGetDevice(&dev));
+    HK_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "hk_voigt_accumulate_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HK_CHECK_ARG(p != nullptr, "hk_voigt_accumulate_noalloc: no device address for scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return HK_OK;
+}
+}  // namespace
+
+extern "C" int hk_voigt_accumulate_noalloc(const hk_line_state *st, const hk_grid *grid,
+                                           double kmin, int accumulate, int mode,
+                                           const hk_tune *tune, double *K, hk_stream_t stream) {
+    HK_CHECK_ARG(st && grid && K, "hk_voigt_accumulate_noalloc: null argument");
+    HK_CHECK_ARG(grid->nx <= HK_VOIGT_NOALLOC_MAX_NX,
+                 "hk_voigt_accumulate_noalloc: nx %lld exceeds the static scratch limit of %lld "
+                 "points (use hk_voigt_accumulate with a workspace)",
+                 (long long)grid->nx, (long long)HK_VOIGT_NOALLOC_MAX_NX);
+    const size_t need = hk_voigt_workspace_bytes(grid, tune);
+    HK_CHECK_ARG(need <= HK_VOIGT_NOALLOC_SCRATCH_BYTES,
+                 "hk_voigt_accumulate_noalloc: %zu scratch bytes needed, %zu available", need,
+                 (size_t)HK_VOIGT_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    if (int e = scratch_for_current_device(&ws)) return e;
+    return hk_voigt_accumulate(st, grid, kmin, accumulate, mode, tune, K, ws,
+                               HK_VOIGT_NOALLOC_SCRATCH_BYTES, stream);
+}
+
+// ---------------------------------------------------------------------------
 // Batched: workspace = [VArgs x nspec, main pass | VArgs x nspec, core pass |
 //   item_start x (nspec+1), main | item_start x (nspec+1), core | 2 queue
 //   counters | per spectrum: main tile table, core tile table]

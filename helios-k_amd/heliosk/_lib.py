@@ -113,6 +113,8 @@ def lib():
         L.hk_voigt_workspace_bytes.argtypes = [P(Grid), P(Tune)]
         L.hk_voigt_accumulate.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int, P(Tune),
                                           c_vp, c_vp, c_size, c_vp]
+        L.hk_voigt_accumulate_noalloc.argtypes = [P(LineState), P(Grid), c_d, c_int, c_int,
+                                                  P(Tune), c_vp, c_vp]
         L.hk_voigt_pair_count.argtypes = [P(LineState), P(Grid), c_vp, c_vp]
         L.hk_voigt_work_count.argtypes = [P(LineState), P(Grid), c_int, P(Tune), c_vp, c_vp]
         L.hk_voigt_autotune.argtypes = [P(LineState), P(Grid), c_int, c_vp, c_vp, c_size, P(Tune),
@@ -159,7 +161,7 @@ EXPORTED = ["hk_last_error", "hk_version", "hk_line_state_bytes", "hk_line_state
             "hk_means", "hk_transmission", "hk_profile_workspace_bytes", "hk_profile_accumulate",
             "hk_means_x", "hk_sort_bins_workspace_bytes", "hk_sort_bins",
             "hk_line_params", "hk_voigt_workspace_bytes", "hk_voigt_accumulate",
-"hk_voigt_pair_count",
+            "hk_voigt_accumulate_noalloc", "hk_voigt_pair_count",
             "hk_voigt_work_count",
             "hk_voigt_autotune", "hk_voigt_autotune_reset",
             "hk_voigt_batched_workspace_bytes", "hk_voigt_batched", "hk_voigt_batched_prepare",

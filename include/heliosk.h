@@ -195,6 +195,22 @@ int hk_voigt_accumulate(const hk_line_state *st, const hk_grid *grid,
                         void *workspace, size_t workspace_bytes,
                         hk_stream_t stream);
 
+/* hk_voigt_accumulate with no workspace argument and no device allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The tile tables live in a static device array of
+ * HK_VOIGT_NOALLOC_SCRATCH_BYTES held by every GPU the library runs on until
+ * the process ends; grid->nx may be at most HK_VOIGT_NOALLOC_MAX_NX (larger
+ * grids return HK_ERR_ARG).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other host threads) race and
+ * must be serialised by the caller.  Same kernels and results as
+ * hk_voigt_accumulate, bit for bit. */
+#define HK_VOIGT_NOALLOC_MAX_NX ((int64_t)1 << 25)
+#define HK_VOIGT_NOALLOC_SCRATCH_BYTES (((size_t)1 << 23) + 512)
+int hk_voigt_accumulate_noalloc(const hk_line_state *st, const hk_grid *grid,
+                                double kmin, int accumulate, int mode,
+                                const hk_tune *tune, double *K, hk_stream_t stream);
+
 /* Algorithmic work of one Voigt sum: counts[0] = line-point pairs inside
  * the cut windows (the evaluations), counts[1..3] = those with
  * x^2+y^2 >= 1e6, in [100, 1e6), < 100 (the reference's regions A, B, C,
