@@ -59,15 +59,18 @@ CLOCK_GHZ = 2.4
 N_CU = 256
 # Kernels are named by the prefix of their demangled name as rocprofv3 prints it.
 # One FP32C Voigt sum of the single-spectrum form at the default 1024-point tiles.
-VOIGT_KERNELS_SINGLE = ("void hk::voigt_tiles_kernel<1, 4>(", "void hk::voigt_core_kernel<1>(")
-# The batched form's two persistent kernels (main pass at tile 512..4096, line-core pass).
-PERSISTENT_MAIN = "void hk::voigt_persistent_kernel<1, %d>("
-PERSISTENT_CORE = "void hk::voigt_persistent_kernel<1, 0>("
+VOIGT_KERNELS_SINGLE = ("void hk::voigt_tiles_kernel<1, 4, 1>(", "void hk::voigt_core_kernel<1>(")
+# The batched form's two persistent kernels: main pass at tile 512..4096 points
+# (256 x points per thread) with 256 or 512 lines per LDS stage (256 x stages), and
+# the line-core pass.
+PERSISTENT_MAIN = "void hk::voigt_persistent_kernel<1, %d, %d>("
+PERSISTENT_CORE = "void hk::voigt_persistent_kernel<1, 0, 1>("
 # The k-table stage: bins of at most 16384 points are sorted and fitted by one
 # kernel; longer bins by LDS-sorted runs, splitters, buckets and a combine.
 KTABLE_KERNELS = ("(anonymous namespace)::build_segs_kernel(",
                   "hk::bin_sort_fit_kernel(", "hk::run_sort_kernel(", "hk::merge_lds_kernel(",
                   "hk::splitter_kernel(", "hk::bucket_kernel(", "hk::bucket_combine_kernel(")
+WAVE = 64
 FLOP_COUNTERS = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP64",
                  "SQ_INSTS_VALU_FLOPS_FP32_TRANS", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
 
@@ -129,7 +132,10 @@ def counter_flops(kernels, workload):
         return None
     out = [0.0, 0.0, 0.0, 0.0]
     for k in ks:
-        out[0] += sum(k[c] for c in FLOP_COUNTERS)
+        # the FLOPS counters count per wave64 instruction (1 per add or mul, 2
+        # per FMA): lane FLOPs are 64 x the count (the 1 GiB FP64 reduction of
+        # the copy probe, 1.34e8 adds, reads 2.12e6)
+        out[0] += WAVE * sum(k[c] for c in FLOP_COUNTERS)
         out[1] += k["SQ_INSTS_VALU"]
         out[2] += k["SQ_ACTIVE_INST_VALU"]
         out[3] += k["SQ_BUSY_CU_CYCLES"]
@@ -152,8 +158,10 @@ def roofline_valu(kernels, workload, ms, label):
             "active_inst_valu_per_busy_cu_cycle": active / busy if busy else None,
             "traffic": hbm_traffic(kernels, workload),
             "kernel": label, "flops_per_launch": flops,
-            "source": "SQ_INSTS_VALU_FLOPS_FP32 + FP64 + FP32_TRANS + FP64_TRANS per launch "
-                      "(profiles/valu_counters_%s.md) / launch time measured here" % workload}
+            "source": "64 lanes x (SQ_INSTS_VALU_FLOPS_FP32 + FP64 + FP32_TRANS + FP64_TRANS) "
+                      "per launch (profiles/valu_counters_%s.md; the counters count per wave "
+                      "instruction, lanes masked off included) / launch time measured here"
+                      % workload}
 
 
 def omp_threads():
@@ -569,11 +577,12 @@ def bench_batched(args, w, wname):
     cpu = None
     if not args.no_cpu:
         i0 = len(w.tp) // 2
-        nspec_kt = max(1, -(-nspec * w.nbins // 100) // w.nbins)    # >= 1 % of the bins
+        nbins_kt = -(-nspec * w.nbins // 100)                       # >= 1 % of the bins
+        nspec_kt = max(1, -(-nbins_kt // w.nbins))
         cpu = cpu_baseline_batched(w, arrays, grid, i0, Kall[:nspec_kt].cpu().numpy(),
                                    coeff64[:nspec_kt].cpu().numpy() if kt_cmp else None)
 
-    main_k = PERSISTENT_MAIN % (tile // 256)
+    main_k = PERSISTENT_MAIN % (tile // 256, stage // 256)
     roof = roofline_valu((main_k, PERSISTENT_CORE), wname, voigt_ms,
                          "batched FP32C Voigt sum = persistent main pass + persistent line-core "
                          "pass (one replay over all %d spectra)" % nspec)
