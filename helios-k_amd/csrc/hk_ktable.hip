@@ -583,13 +583,16 @@ __global__ __launch_bounds__(128) void bucket_combine_kernel(FitArgs fa, int64_t
 // one spectrum.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void kfit_literal_kernel(const double *__restrict__ Ksorted,
-                                                           int64_t nx, int nbins, FitArgs fa) {
+                                                           int64_t nx, int nbins, int nspec,
+                                                           FitArgs fa) {
     __shared__ double red[4][kMomGroup];
     __shared__ double bl[kHHMaxNxb];
     __shared__ double mom[kMaxNC];
     __shared__ double b0[kMaxNC];
     __shared__ int s_r1[256], s_r2[256], s_r3[256];
-    const int b = blockIdx.x, sp = blockIdx.y, t = threadIdx.x;
+    const int64_t f = flat_block();
+    if (f >= (int64_t)nbins * nspec) return;
+    const int b = (int)(f % nbins), sp = (int)(f / nbins), t = threadIdx.x;
     const int Nxb = fa.Nxb, nC = fa.nC;
     const double *Ks = Ksorted + (int64_t)sp * nx;
     const double *Kb = Ks + (int64_t)b * Nxb;
@@ -1326,8 +1329,8 @@ extern "C" int hk_ktable(const double *K, int nspec, const hk_grid *g, double nu
     if (int e = sort_fit_segments(K, segs, nseg, SegMap{nbins + 1, 0}, L.sp, L.sl, base, fa, sorted,
                                   s))
         return e;
-    hipLaunchKernelGGL(kfit_literal_kernel, dim3(nbins, nspec), dim3(256), 0, s, sorted, g->nx,
-                       nbins, fa);
+    hipLaunchKernelGGL(kfit_literal_kernel, flat_grid((int64_t)nbins * nspec), dim3(256), 0, s,
+                       sorted, g->nx, nbins, nspec, fa);
     HK_LAUNCH_CHECK();
     return HK_OK;
 }

@@ -424,17 +424,31 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
     }
 }
 
-template <int PPT>
-__device__ __forceinline__ void dispatch32(const Rec32 &q, int code, float *part, int jr0) {
-    switch (code) {
-        case 1: eval32<kPathFar, true, PPT>(q, part, jr0); break;
-        case 2: eval32<kPathFar, false, PPT>(q, part, jr0); break;
-        case 3: eval32<kPathMid, true, PPT>(q, part, jr0); break;
-        case 4: eval32<kPathMid, false, PPT>(q, part, jr0); break;
-        case 5: eval32<kPathMid2, true, PPT>(q, part, jr0); break;
-        case 6: eval32<kPathMid2, false, PPT>(q, part, jr0); break;
-        case 7: eval32<kPathAny, true, PPT>(q, part, jr0); break;
-        default: eval32<kPathAny, false, PPT>(q, part, jr0); break;
+// The records of one path code that this wave takes (mask m over records
+// c0 .. c0 + 63), in ascending order, with the next record's LDS fields
+// loaded while the current one is evaluated (the loads are the latency of a
+// record; an eval of the cheap paths is a few dozen instructions).
+template <int PATH, bool FULL, int PPT>
+__device__ __forceinline__ void run_path(const Rec32 *rec, int c0, unsigned long long m,
+                                         float *part, float *hi, float *lo, int &since,
+                                         int jr0) {
+    if (!m) return;
+    Rec32 cur = rec[c0 + lowest_bit(m)];
+    // the first record's loads complete here, so that inside the loop only
+    // the copy of the next record waits (after the current evaluation)
+    __builtin_amdgcn_s_waitcnt(0xc07f);      // lgkmcnt(0)
+    for (;;) {
+        const bool more = m != 0ull;
+        Rec32 nxt;
+        if (more) nxt = rec[c0 + lowest_bit(m)];
+        eval32<PATH, FULL, PPT>(cur, part, jr0);
+        if (++since == 32) {
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
+            since = 0;
+        }
+        if (!more) break;
+        cur = nxt;
     }
 }
 
@@ -509,21 +523,29 @@ __device__ __forceinline__ double far_u(int k) {
     return N == kFN1 ? kFarU1[k] : (N == kFN2 ? kFarU2[k] : kFarU3[k]);
 }
 
-// One band's per-wave node sums (FP32, compensated: hi + lo), their totals
-// and the Chebyshev coefficients.
+// One band's node sums (FP32, compensated: hi + lo), their totals and the
+// Chebyshev coefficients.  Each wave keeps its own column of sums, so no two
+// waves write one word.  Lines that reach every point of the tile add to the
+// common columns (hi, lo); a line whose window ends inside the tile adds only
+// to the targets (the waves whose points it covers) in the target columns
+// (xhi, xlo).  Totals and coefficients are per target wave: each wave
+// interpolates its own sums at its own points.
+constexpr int kWaves = kThreads / 64;
 template <int N>
 struct FarBand {
-    float hi[N][kThreads / 64];       // node sums, one column per wave
-    float lo[N][kThreads / 64];
-    double F[N];                      // node totals
-    double c[N];                      // Chebyshev coefficients
+    float hi[N][kWaves];              // common node sums, one column per wave
+    float lo[N][kWaves];
+    float xhi[N][kWaves][kWaves];     // [node][target][wave]
+    float xlo[N][kWaves][kWaves];
+    double F[kWaves][N];              // node totals per target wave
+    double c[kWaves][N];              // Chebyshev coefficients per target wave
 };
 
 struct FarLDS {
     FarBand<kFN1> b1;
     FarBand<kFN2> b2;
     FarBand<kFN3> b3;
-    int cnt[2][kThreads / 64];        // compaction counts, alternate stages
+    int cnt[2][kWaves];               // compaction counts, alternate stages
     int any1, any2, any3;             // band used (set to 1 by any wave)
 };
 
@@ -583,103 +605,184 @@ __device__ __forceinline__ float wave_total32(float v) {
 
 // Far-field band (0: evaluated point by point) of a line (centre nu, cut
 // half width hw, 1/alpha_D ia, y) for the tile [j0, j0 + Wv) with half
-// width hW (points): its window covers the tile (the cut predicate
-// |x_j - nu| < hw holds at both end points, so at all of them), its centre
-// is at least kFarRatio3 half widths from the tile's centre (band 2 from
-// kFarRatio2 on, band 1 from kFarRatio1 on) and x^2 + y^2 >= kFarR2 at the
-// nearest point.  No division,
-// so it is cheap enough to run for every staged line before its window is
-// computed.
+// width hW (points): its centre is at least kFarRatio3 half widths from the
+// tile's centre (band 2 from kFarRatio2 on, band 1 from kFarRatio1 on) and
+// x^2 + y^2 >= kFarR2 at the nearest point.  full: the window covers the
+// tile (the cut predicate |x_j - nu| < hw holds at both end points, so at
+// all of them).  No division, so it is cheap enough to run for every staged
+// line before its window is computed.
 __device__ __forceinline__ int far_band(double nu, double hw, double ia, double y, int64_t j0,
-                                        int64_t Wv, double hW, double numin, double dnu) {
+                                        int64_t Wv, double hW, double numin, double dnu,
+                                        bool &full) {
+    full = false;
     if (Wv < kFarMinPts || !(hw > 0.0)) return 0;
-    const double dlo = fma((double)j0, dnu, numin) - nu;
-    const double dhi = fma((double)(j0 + Wv - 1), dnu, numin) - nu;
-    if (!(dlo > -hw && dlo < hw && dhi > -hw && dhi < hw)) return 0;
     const double half = hW * dnu;
     const double d = fabs(nu - fma((double)j0 + hW, dnu, numin));
     if (d < kFarRatio3 * half) return 0;
     const double xnear = (d - half) * ia;
     if (!(fma(xnear, xnear, y * y) >= kFarR2)) return 0;
+    const double dlo = fma((double)j0, dnu, numin) - nu;
+    const double dhi = fma((double)(j0 + Wv - 1), dnu, numin) - nu;
+    full = dlo > -hw && dlo < hw && dhi > -hw && dhi < hw;
     return d >= kFarRatio1 * half ? 1 : (d >= kFarRatio2 * half ? 2 : 3);
+}
+
+// Line offsets of the FP32 records: jc = the grid point nearest the centre
+// (may lie off the grid), rr = (x_jc - nu) / alpha_D, vb = dnu / alpha_D and
+// d0 = j0 - jc, so that x at tile-relative point j is (j + d0) vb + rr.
+__device__ __forceinline__ void line_offsets(double nu, double ia, double numin, double dnu,
+                                             int64_t j0, double &rr, double &vb, double &d0) {
+    const double jcd = rint((nu - numin) / dnu);
+    rr = (fma(jcd, dnu, numin) - nu) * ia;
+    vb = dnu * ia;
+    d0 = (double)j0 - jcd;
+}
+
+// A far line whose window ends inside the tile: the window's tile-relative
+// part [lo, hi) (valid points, Wv of them) is split into the waves it covers
+// (wave w owns [w wpts, (w + 1) wpts)), which take the line from the far
+// field, and the rest, which stays with the pair-by-pair path: lo and hi are
+// moved past the covered waves.  Returns the covered-wave mask with the waves
+// that own no valid point set too (mask 0xF: the far field takes every
+// point); 0 when no wave with valid points is covered.  The centre lies
+// outside the tile, so the window reaches one end of it and the covered waves
+// are a run at that end: [lo, hi) stays one interval.
+__device__ __forceinline__ int far_cover(int32_t &lo, int32_t &hi, int Wv, int wpts) {
+    int mask = 0, vmask = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int a = w * wpts, b = min(a + wpts, Wv);
+        if (a >= Wv) mask |= 1 << w;
+        else {
+            vmask |= 1 << w;
+            if (lo <= a && hi >= b) mask |= 1 << w;
+        }
+    }
+    if (!(mask & vmask)) return 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        if (lo < hi && ((mask >> (lo / wpts)) & 1)) lo = min((lo / wpts + 1) * wpts, hi);
+        if (lo < hi && ((mask >> ((hi - 1) / wpts)) & 1)) hi = max(((hi - 1) / wpts) * wpts, lo);
+    }
+    return mask;
 }
 
 template <int N>
 __device__ __forceinline__ void band_init(FarBand<N> *b, int t) {
-    if (t < N * (kThreads / 64)) {
-        b->hi[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0f;
-        b->lo[t / (kThreads / 64)][t % (kThreads / 64)] = 0.0f;
+    for (int e = t; e < N * kWaves; e += kThreads) {
+        (&b->hi[0][0])[e] = 0.0f;
+        (&b->lo[0][0])[e] = 0.0f;
+    }
+    for (int e = t; e < N * kWaves * kWaves; e += kThreads) {
+        (&b->xhi[0][0][0])[e] = 0.0f;
+        (&b->xlo[0][0][0])[e] = 0.0f;
     }
 }
 
 // The wave's lines of this band (mine) add their node values to the wave's
-// column of node sums (all lanes take part in the wave sums).  A line's
+// columns of node sums (all lanes take part in the wave sums).  A line's
 // Doppler-unit offset from the tile centre D and the tile half width h come
 // in FP64 (one per line and tile); each node's x = u_k h + D is rounded to
 // FP32 once, and the series, the strength and the wave sum are FP32.  The
-// column sums over the stages are compensated (two-sum, hi + lo).
+// column sums over the stages are compensated (two-sum, hi + lo).  When a
+// line of the wave covers only some waves (cover != 0xF), the wave sums go
+// per target wave into the target columns instead (four wave sums a node).
 template <int N>
-__device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, double D, double h,
+__device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, double D, double h,
                                            float fy, float fB, int lane, int wave) {
+    const bool split = __ballot(mine && cover != 0xF) != 0ull;
+    // x^2 + y^2 >= kRF32 at every node of every line of the wave: the
+    // two-term series of the pair-by-pair far path (truncation < 5e-8)
+    // instead of the four-term complex one
+    const double xn = fabs(D) - h;
+    const bool series2 = __ballot(mine && !(fma(xn, xn, (double)fy * fy) >= (double)kRF32)) == 0ull;
+    const float y2 = fy * fy, B = fB * fy * (float)(1.0 / kPi);
+    const float B15 = 1.5f * B, BC = fmaxf(-2.0f * y2 * B, -1.0e37f);
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
         const float x = (float)fma(far_u<N>(k), h, D);
-        const float v = mine ? fB * far_node32(x, fy) : 0.0f;
-        const float sw = wave_total32(v);
-        if (lane == 63) {
-            float hi = b->hi[k][wave], lo = b->lo[k][wave];
-            two_sum_acc(hi, lo, sw);
-            b->hi[k][wave] = hi;
-            b->lo[k][wave] = lo;
+        float v = 0.0f;
+        if (series2) {
+            const float inv = rcp32(fmaf(x, x, y2));
+            v = mine ? inv * fmaf(fmaf(BC, inv, B15), inv, B) : 0.0f;
+        } else {
+            v = mine ? fB * far_node32(x, fy) : 0.0f;
+        }
+        if (!split) {
+            const float sw = wave_total32(v);
+            if (lane == 63) {
+                float hi = b->hi[k][wave], lo = b->lo[k][wave];
+                two_sum_acc(hi, lo, sw);
+                b->hi[k][wave] = hi;
+                b->lo[k][wave] = lo;
+            }
+        } else {
+#pragma unroll
+            for (int tw = 0; tw < kWaves; ++tw) {
+                const float sw = wave_total32((cover >> tw) & 1 ? v : 0.0f);
+                if (lane == 63) {
+                    float hi = b->xhi[k][tw][wave], lo = b->xlo[k][tw][wave];
+                    two_sum_acc(hi, lo, sw);
+                    b->xhi[k][tw][wave] = hi;
+                    b->xlo[k][tw][wave] = lo;
+                }
+            }
         }
     }
 }
 
-// Node totals of the waves -> Chebyshev coefficients, by threads t0 .. t0 + N
-// - 1 (two barriers are taken by the caller between the phases).  The
-// interpolation -- totals, coefficients and the per-point Clenshaw sums --
-// runs in FP64: a band-3 tile's far field varies by up to 25x across it, and
-// an FP32 Clenshaw sum would carry up to ~3e-5 of the smallest value.
+// Node totals -> Chebyshev coefficients, per target wave, by threads t0 ..
+// t0 + 4N - 1 (two barriers are taken by the caller between the phases).
+// The interpolation -- totals, coefficients and the per-point Clenshaw sums
+// -- runs in FP64: a band-3 tile's far field varies by up to 25x across it,
+// and an FP32 Clenshaw sum would carry up to ~3e-5 of the smallest value.
 template <int N>
 __device__ __forceinline__ void band_sum(FarBand<N> *b, int t) {
-    if (t >= 0 && t < N) {
+    if (t >= 0 && t < N * kWaves) {
+        const int tw = t / N, k = t % N;
         double F = 0.0;
 #pragma unroll
-        for (int w = 0; w < kThreads / 64; ++w) F += (double)b->hi[t][w] + (double)b->lo[t][w];
-        b->F[t] = F;
+        for (int w = 0; w < kWaves; ++w) F += (double)b->hi[k][w] + (double)b->lo[k][w];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) F += (double)b->xhi[k][tw][w] + (double)b->xlo[k][tw][w];
+        b->F[tw][k] = F;
     }
 }
 
 template <int N>
 __device__ __forceinline__ void band_coeffs(FarBand<N> *b, int t) {
-    if (t >= 0 && t < N) {
-        // c_t = (2 - [t == 0]) / N sum_k F_k T_t(u_k), T_t by its recurrence
+    if (t >= 0 && t < N * kWaves) {
+        const int tw = t / N, m = t % N;
+        // c_m = (2 - [m == 0]) / N sum_k F_k T_m(u_k), T_m by its recurrence
+        // (not unrolled: unrolled, the node constants are hoisted out of the
+        // persistent kernel's item loop and held in registers throughout)
         double c = 0.0;
+#pragma unroll 1
         for (int k = 0; k < N; ++k) {
             const double u = far_u<N>(k);
             double tm = 1.0, tc = u;
-            for (int m = 2; m <= t; ++m) {
+            for (int q = 2; q <= m; ++q) {
                 const double tn = fma(2.0 * u, tc, -tm);
                 tm = tc;
                 tc = tn;
             }
-            c += b->F[k] * (t == 0 ? 1.0 : tc);
+            c += b->F[tw][k] * (m == 0 ? 1.0 : tc);
         }
-        b->c[t] = (t == 0 ? 1.0 : 2.0) / (double)N * c;
+        b->c[tw][m] = (m == 0 ? 1.0 : 2.0) / (double)N * c;
     }
 }
 
-// Clenshaw sum of the band's interpolant at u in [-1, 1].
+// Clenshaw sum of an N-term Chebyshev series at u in [-1, 1].
 template <int N>
-__device__ __forceinline__ double band_eval(const FarBand<N> *b, double u) {
+__device__ __forceinline__ double cheb_eval(const double *c, double u) {
     double b1 = 0.0, b2 = 0.0;
 #pragma unroll
     for (int m = N - 1; m >= 1; --m) {
-        const double b0 = fma(2.0 * u, b1, b->c[m] - b2);
+        const double b0 = fma(2.0 * u, b1, c[m] - b2);
         b2 = b1;
         b1 = b0;
     }
-    return fma(u, b1, b->c[0] - b2);
+    return fma(u, b1, c[0] - b2);
 }
 
 template <int PPT, int LM>
@@ -714,48 +817,59 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         const int64_t i = base + q * kThreads + t;
         Rec32 r;
         r.lo = 0; r.hi = 0; r.code = 0;
-        bool far = false;
+        // the line's window on the tile, [clo, chi) tile-relative, less the
+        // waves the far field takes
+        int32_t clo = 0, chi = 0;
         // far lines first: FP32 values at the nodes, summed per wave (their
         // registers are free again before the direct records are built)
-        if (ffok) {
+        {
             double fD = 0.0, fh = 0.0;
             float fy = 0.0f, fB = 0.0f;
-            int band = 0;
+            int band = 0, cover = 0;
             if (i < ti.hi) {
-                const double fnu = a.nu[i], fia = a.ia[i], y = a.y[i];
-                band = far_band(fnu, a.hw[i], fia, y, j0, Wv, hW, a.numin, a.dnu);
+                const double fnu = a.nu[i], fhw = a.hw[i];
+                bool full = false;
+                if (ffok) band = far_band(fnu, fhw, a.ia[i], a.y[i], j0, Wv, hW, a.numin, a.dnu, full);
+                if (band && full) {
+                    cover = 0xF;
+                } else {
+                    int64_t wl, wh;
+                    cut_window(fnu, fhw, a.numin, a.dnu, a.nx, wl, wh);
+                    clo = (int32_t)(max(wl, j0) - j0);
+                    chi = (int32_t)(min(wh, j0 + W) - j0);
+                    if (band) {
+                        cover = clo < chi ? far_cover(clo, chi, (int)Wv, 64 * PPT) : 0;
+                        if (!cover) band = 0;
+                    }
+                }
                 if (band) {
+                    const double fia = a.ia[i];
                     fD = (xc - fnu) * fia;
                     fh = hW * a.dnu * fia;
-                    fy = (float)y;
+                    fy = (float)a.y[i];
                     fB = (float)ldexp(a.S[i], -ti.e);
                 }
             }
-            far = band != 0;
             if (__ballot(band == 1) != 0ull) {
-                band_nodes<kFN1>(&fl->b1, band == 1, fD, fh, fy, fB, lane, wave);
+                band_nodes<kFN1>(&fl->b1, band == 1, cover, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any1 = 1;
             }
             if (__ballot(band == 2) != 0ull) {
-                band_nodes<kFN2>(&fl->b2, band == 2, fD, fh, fy, fB, lane, wave);
+                band_nodes<kFN2>(&fl->b2, band == 2, cover, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any2 = 1;
             }
             if (__ballot(band == 3) != 0ull) {
-                band_nodes<kFN3>(&fl->b3, band == 3, fD, fh, fy, fB, lane, wave);
+                band_nodes<kFN3>(&fl->b3, band == 3, cover, fD, fh, fy, fB, lane, wave);
                 if (lane == 0) fl->any3 = 1;
             }
         }
-        if (i < ti.hi && !far) {
+        if (clo < chi) {
             const double nu = a.nu[i];
-            int64_t wl, wh;
-            cut_window(nu, a.hw[i], a.numin, a.dnu, a.nx, wl, wh);
-            const int64_t clo = max(wl, j0), chi = min(wh, j0 + W);
-            if (clo < chi) {
+            {
                 const double ia = a.ia[i], y = a.y[i];
                 // nearest grid point to the centre (may lie off the grid)
-                const double jcd = rint((nu - a.numin) / a.dnu);
-                const double rr = (fma(jcd, a.dnu, a.numin) - nu) * ia;
-                const double vb = a.dnu * ia, d0 = (double)j0 - jcd;
+                double rr, vb, d0;
+                line_offsets(nu, ia, a.numin, a.dnu, j0, rr, vb, d0);
                 const float vbh = (float)vb;
                 const double A = ldexp(a.S[i], -ti.e) * kISqrtPi;
                 const float yf = (float)y;
@@ -780,8 +894,8 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 r.q5 = (float)(210.0 * yy * yy * Bm);
                 r.q6 = (float)(-120.0 * yy * yy * yy * Bm);
                 r.d0 = (int32_t)d0;
-                r.lo = (int32_t)(clo - j0);
-                r.hi = (int32_t)(chi - j0);
+                r.lo = clo;
+                r.hi = chi;
                 r.dop = yf < kY0 ? 1 : 0;
                 const double y2 = (double)yf * (double)yf;
                 int32_t ri[6];
@@ -814,31 +928,30 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         nprev = nst;
         for (int c0 = 0; c0 < nst; c0 += 64) {
             const int nib = c0 + lane < nst ? (rec[c0 + lane].code >> (4 * wave)) & 15 : 0;
-            unsigned long long m = __ballot(nib != 0);       // 0: window misses this wave
-            while (m) {
-                const int b = lowest_bit(m);
-                const int code = __builtin_amdgcn_readlane(nib, b);
-                dispatch32<PPT>(rec[c0 + b], code, part, jr0);
-                if (++since == 32) {
-#pragma unroll
-                    for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
-                    since = 0;
-                }
-            }
+            // 0: the window misses this wave; otherwise wave_codes' path code
+            // (1 + 2 path + partial), taken path by path
+            run_path<kPathFar, true, PPT>(rec, c0, __ballot(nib == 1), part, hi, lo, since, jr0);
+            run_path<kPathFar, false, PPT>(rec, c0, __ballot(nib == 2), part, hi, lo, since, jr0);
+            run_path<kPathMid, true, PPT>(rec, c0, __ballot(nib == 3), part, hi, lo, since, jr0);
+            run_path<kPathMid, false, PPT>(rec, c0, __ballot(nib == 4), part, hi, lo, since, jr0);
+            run_path<kPathMid2, true, PPT>(rec, c0, __ballot(nib == 5), part, hi, lo, since, jr0);
+            run_path<kPathMid2, false, PPT>(rec, c0, __ballot(nib == 6), part, hi, lo, since, jr0);
+            run_path<kPathAny, true, PPT>(rec, c0, __ballot(nib == 7), part, hi, lo, since, jr0);
+            run_path<kPathAny, false, PPT>(rec, c0, __ballot(nib == 8), part, hi, lo, since, jr0);
         }
     }
     // far field: node sums -> Chebyshev coefficients -> the tile's points
     __syncthreads();
     const int any = fl->any1 | (fl->any2 << 1) | (fl->any3 << 2);
     if (any) {
-        // band 1 on threads 0..8, band 2 on 64..78, band 3 on 128..146
+        // band 1 on threads 0..35, band 2 on 40..99, band 3 on 100..175
         band_sum<kFN1>(&fl->b1, t);
-        band_sum<kFN2>(&fl->b2, t - 64);
-        band_sum<kFN3>(&fl->b3, t - 128);
+        band_sum<kFN2>(&fl->b2, t - 40);
+        band_sum<kFN3>(&fl->b3, t - 100);
         __syncthreads();
         band_coeffs<kFN1>(&fl->b1, t);
-        band_coeffs<kFN2>(&fl->b2, t - 64);
-        band_coeffs<kFN3>(&fl->b3, t - 128);
+        band_coeffs<kFN2>(&fl->b2, t - 40);
+        band_coeffs<kFN3>(&fl->b3, t - 100);
         __syncthreads();
     }
 #pragma unroll
@@ -849,9 +962,9 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         if (any) {
             // the interpolants at u = (jr - hW) / hW
             const double u = ((double)(jr0 + 64 * k) - hW) / hW;
-            if (any & 1) ff += band_eval<kFN1>(&fl->b1, u);
-            if (any & 2) ff += band_eval<kFN2>(&fl->b2, u);
-            if (any & 4) ff += band_eval<kFN3>(&fl->b3, u);
+            if (any & 1) ff += cheb_eval<kFN1>(fl->b1.c[wave], u);
+            if (any & 2) ff += cheb_eval<kFN2>(fl->b2.c[wave], u);
+            if (any & 4) ff += cheb_eval<kFN3>(fl->b3.c[wave], u);
         }
         if (j < a.nx) {
             const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
@@ -1149,12 +1262,15 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                                                          const double *__restrict__ ia,
                                                          const double *__restrict__ y, int64_t nl,
                                                          double numin, double dnu, int64_t nx,
-                                                         int W, int far,
+                                                         int ppt, int far,
                                                          unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long red[7][256];
+    constexpr int kOut = 7;
+    __shared__ unsigned long long red[kOut][256];
     const int t = threadIdx.x;
+    const int W = kThreads * ppt, WP = 64 * ppt;
     const int64_t i = (int64_t)blockIdx.x * 256 + t;
-    unsigned long long cA = 0, cB = 0, cC = 0, fp = 0, fl = 0, fl2 = 0, fl3 = 0;
+    // direct A, B, C; far-field pairs; far (line, tile) pairs of bands 1-3
+    unsigned long long c[kOut] = {};
     if (i < nl) {
         int64_t wl, wh, bl = 0, bh = 0, cl = 0, ch = 0;
         cut_window(nu[i], hw[i], numin, dnu, nx, wl, wh);
@@ -1169,30 +1285,35 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
             for (int64_t tl = wl / W; tl <= (wh - 1) / W; ++tl) {
                 const int64_t j0 = tl * W, Wv = min((int64_t)W, nx - j0);
                 const double hW = 0.5 * (double)(Wv - 1);
-                const int band = far ? far_band(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu) : 0;
+                bool full = false;
+                int band = far ? far_band(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu, full) : 0;
+                // the pairs left to the pair-by-pair path: [dlo, dhi)
+                int32_t dlo = (int32_t)(max(wl, j0) - j0), dhi = (int32_t)(min(wh, j0 + Wv) - j0);
+                const int32_t n0 = dhi - dlo;
+                if (band && !full && !far_cover(dlo, dhi, (int)Wv, WP)) band = 0;
                 if (band) {
-                    fp += (unsigned long long)Wv;
-                    if (band == 1) fl += 1; else if (band == 2) fl2 += 1; else fl3 += 1;
-                    continue;
+                    c[3] += (unsigned long long)(full ? Wv : n0 - (dhi - dlo));
+                    c[3 + band] += 1;
+                    if (full) continue;
                 }
-                const unsigned long long tot = ov(wl, wh, j0, j0 + Wv);
-                const unsigned long long bc = ov(bl, bh, j0, j0 + Wv);
-                const unsigned long long c = ov(cl, ch, j0, j0 + Wv);
-                cA += tot - bc;
-                cB += bc - c;
-                cC += c;
+                const int64_t p0 = j0 + dlo, p1 = j0 + dhi;
+                const unsigned long long tot = ov(wl, wh, p0, p1);
+                const unsigned long long bc = ov(bl, bh, p0, p1);
+                const unsigned long long cc = ov(cl, ch, p0, p1);
+                c[0] += tot - bc;
+                c[1] += bc - cc;
+                c[2] += cc;
             }
         }
     }
-    red[0][t] = cA; red[1][t] = cB; red[2][t] = cC; red[3][t] = fp; red[4][t] = fl;
-    red[5][t] = fl2; red[6][t] = fl3;
+    for (int q = 0; q < kOut; ++q) red[q][t] = c[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
-            for (int q = 0; q < 7; ++q) red[q][t] += red[q][t + s];
+            for (int q = 0; q < kOut; ++q) red[q][t] += red[q][t + s];
         __syncthreads();
     }
-    if (t < 7) atomicAdd(&out[t], red[t][0]);
+    if (t < kOut) atomicAdd(&out[t], red[t][0]);
 }
 
 __global__ void zero8_kernel(unsigned long long *c) {
@@ -1691,7 +1812,7 @@ extern "C" int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid,
     if (st->nlines > 0) {
         hipLaunchKernelGGL(work_count_kernel, dim3((unsigned)((st->nlines + 255) / 256)), dim3(256),
                            0, s, st->nu, st->hw, st->ialphaD, st->y, st->nlines, grid->numin,
-                           grid->dnu, grid->nx, kThreads * ppt, far, counts);
+                           grid->dnu, grid->nx, ppt, far, counts);
         HK_LAUNCH_CHECK();
     }
     return HK_OK;

@@ -148,8 +148,10 @@ typedef struct {
                            through its Chebyshev far field */
     int stage_lines;    /* lines staged through LDS per step of the main
                            pass: 256 (default) or 512 (the reference's fixed
-                           line blocks def_nlA/B/C, define.h:56-58); results
-                           do not depend on it */
+                           line blocks def_nlA/B/C, define.h:56-58); FP64
+                           results do not depend on it, FP32C results only
+                           by rounding (a stage's lines are added path by
+                           path, so the order changes with the stage) */
 } hk_tune;
 
 const char *hk_last_error(void);
@@ -223,7 +225,9 @@ int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
  * [direct pairs in regions A, B, C (as hk_voigt_pair_count), pairs summed
  * through the FP32 far field, far (line, tile) pairs of band 1 -- each 9
  * node evaluations --, of band 2 -- 15 each --, of band 3 -- 19 each --,
- * 0].  The direct and far-field pairs add up to counts[0] of
+ * 0].  A far line whose window ends inside a tile counts as a far (line,
+ * tile) pair; its pairs in the wave that holds the window's end are direct.
+ * The direct and far-field pairs add up to counts[0] of
  * hk_voigt_pair_count. */
 int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
                         const hk_tune *tune, unsigned long long *counts, hk_stream_t stream);

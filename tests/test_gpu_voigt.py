@@ -339,6 +339,34 @@ def test_fp32c_far_field_vs_exact_pairs(E, cm, cut, dnu):
     assert np.array_equal(Kff, E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C).cpu().numpy())
 
 
+@pytest.mark.parametrize("tp, cut, n", [(512, 6.0, 20), (1024, 12.0, 10), (4096, 49.0, 4)])
+def test_far_field_window_ends_inside_tile(E, tp, cut, n):
+    """Sparse lines whose cut windows (three tile widths each side) end inside
+    tiles: the waves a far line covers take it from the far field, the wave
+    with its window edge evaluates it pair by pair.  The points outside every
+    window stay exactly zero, the sum matches FP64 and the every-pair form,
+    the work count still adds up, and the batched form gives the same bits."""
+    import torch
+    d = random_lines(n, seed=48, numin=900.0, numax=1500.0, gamma=(0.01, 0.1))
+    st = E.line_params(species_from(d), 1500.0, 1.0, Q=d["iso_Q"], cut_mode=0, cut=cut)
+    grid = E.make_grid(1000.0, 1400.0, 0.004)
+    K64 = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP64).cpu().numpy()
+    Kff = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, tile_points=tp).cpu().numpy()
+    Kex = E.voigt(st, grid, 0.0, E.HK_VOIGT_FP32C, tile_points=tp, exact_pairs=True).cpu().numpy()
+    assert np.array_equal(Kff == 0.0, K64 == 0.0) and np.any(K64 == 0.0)
+    e_ff, d_fe = rel_err(Kff, K64), rel_err(Kff, Kex)
+    tot, A, B, C = E.pair_count(st, grid)
+    w = E.work_count(st, grid, E.HK_VOIGT_FP32C, tile_points=tp)
+    print("tile %d: far field vs FP64 %.3e, vs every pair %.3e; direct %d far %d of %d"
+          % (tp, e_ff, d_fe, sum(w[:3]), w[3], tot))
+    assert e_ff <= 1e-5 and 0.0 < d_fe <= 2e-6
+    assert w[3] > 0 and sum(w[:4]) == tot
+    K = torch.empty(grid.nx, dtype=torch.float64, device="cuda")
+    E.VoigtBatch([(st, grid, 0.0, K)], mode=E.HK_VOIGT_FP32C, tile_points=tp).launch()
+    torch.cuda.synchronize()
+    assert np.array_equal(K.cpu().numpy(), Kff)
+
+
 def test_work_count_adds_up(E):
     """Direct pairs by region plus far-field pairs equal the pair count; with
     every pair evaluated (or in FP64) nothing goes through the far field."""
@@ -512,11 +540,15 @@ def test_side_stream_keeps_transient_buffers(E):
     del Kb
 
 
-def test_stage_lines_do_not_change_results(E):
+def test_stage_lines_change_results_by_rounding_only(E):
     """512 lines per LDS stage against 256, every tile width, both forms,
-    single and batched launches: bit-identical sums (lines are added in the
-    same order), and the autotuner returns a valid (tile, stage) pair that a
-    second tuning of the same problem repeats from its cache."""
+    single and batched launches.  FP64 adds the lines in line order: bit-
+    identical sums.  FP32C takes a stage's records path by path: the order
+    changes with the stage size, so the sums agree to rounding (1e-6
+    relative); the batched and single launches of one (tile, stage) give the
+    same bits.  The autotuner returns
+    a valid (tile, stage) pair that a second tuning of the same problem
+    repeats from its cache."""
     import torch
     d = random_lines(40000, seed=48, numin=900.0, numax=1100.0)
     st = E.line_params(species_from(d), 1700.0, 3.0, Q=d["iso_Q"], cut_mode=0, cut=25.0)
@@ -525,11 +557,14 @@ def test_stage_lines_do_not_change_results(E):
         for tp in (512, 1024, 2048, 4096):
             a = E.voigt(st, grid, 0.0, mode, tile_points=tp, stage_lines=256)
             b = E.voigt(st, grid, 0.0, mode, tile_points=tp, stage_lines=512)
-            assert torch.equal(a, b), (mode, tp)
+            if mode == E.HK_VOIGT_FP64:
+                assert torch.equal(a, b), (mode, tp)
+            else:
+                assert rel_err(b.cpu().numpy(), a.cpu().numpy()) <= 1e-6, (mode, tp)
         K = torch.empty(grid.nx, dtype=torch.float64, device="cuda")
         E.VoigtBatch([(st, grid, 0.0, K)], mode=mode, tile_points=1024, stage_lines=512).launch()
         torch.cuda.synchronize()
-        assert torch.equal(K, E.voigt(st, grid, 0.0, mode, tile_points=1024))
+        assert torch.equal(K, E.voigt(st, grid, 0.0, mode, tile_points=1024, stage_lines=512))
     E.lib().hk_voigt_autotune_reset()
     t1, s1, ms1 = E.autotune(st, grid, E.HK_VOIGT_FP32C)
     t2, s2, ms2 = E.autotune(st, grid, E.HK_VOIGT_FP32C)
