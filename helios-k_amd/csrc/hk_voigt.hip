@@ -694,13 +694,16 @@ __device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, 
     // x^2 + y^2 >= kRF32 at every node of every line of the wave: the
     // two-term series of the pair-by-pair far path (truncation < 5e-8)
     // instead of the four-term complex one
+    const float fD = (float)D, fh = (float)h;
     const double xn = fabs(D) - h;
     const bool series2 = __ballot(mine && !(fma(xn, xn, (double)fy * fy) >= (double)kRF32)) == 0ull;
     const float y2 = fy * fy, B = fB * fy * (float)(1.0 / kPi);
     const float B15 = 1.5f * B, BC = fmaxf(-2.0f * y2 * B, -1.0e37f);
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
-        const float x = (float)fma(far_u<N>(k), h, D);
+        // far lines: |x| >= |D| - h >= |D| / 3, so x in FP32 from FP32 D and
+        // h keeps ~1e-7 of x
+        const float x = fmaf((float)far_u<N>(k), fh, fD);
         float v = 0.0f;
         if (series2) {
             const float inv = rcp32(fmaf(x, x, y2));
@@ -1127,8 +1130,10 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
 // XCDs balanced; giving each XCD a contiguous range instead (better L2 reuse
 // of line data) measured 1.1x slower for the main pass and 2x for the
 // line-core pass.
+constexpr int main_waves(int mode, int ppt, int lm);
+
 template <int MODE, int PPT, int LM>
-__global__ __launch_bounds__(256) void voigt_tiles_kernel(VArgs a) {
+__global__ __launch_bounds__(256, main_waves(MODE, PPT, LM)) void voigt_tiles_kernel(VArgs a) {
     const int64_t tile = blockIdx.x;
     if (MODE == HK_VOIGT_FP64) {
         __shared__ Rec64 rec[kThreads * LM];
@@ -1157,13 +1162,22 @@ __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
 // Persistent form: workgroups take (spectrum, tile) items from a queue.
 // item_start[s] = first item of spectrum s (nspec+1 entries).  PPT == 0
 // selects the line-core pass.
-// Occupancy floors: the main pass 4 waves per SIMD for tiles up to 1024
-// points (3 with 512-line stages, whose LDS allows 3 workgroups per CU), 3
-// for 2048 (without them the far-field code raises the register count of
-// the persistent form enough to halve its occupancy), the FP32 line-core
-// pass 5 (as its one-spectrum kernel has without a floor).
+// Occupancy floors (waves per SIMD): the FP32C main pass is latency-bound
+// (LDS record reads, staging loads, a barrier per stage), so occupancy beats
+// registers: 5 for tiles up to 1024 points with 256-line stages (LDS allows
+// 5 workgroups per CU; the few values that then spill are reloaded once per
+// stage or per tile, outside the record loops: c2's batched sum 1070 -> 964
+// ms against 4), 3 with 512-line stages (LDS allows 3 workgroups per CU) and
+// for 2048-point tiles (without a floor the far-field code raises the
+// register count of the persistent form enough to halve its occupancy); the
+// FP64 main pass 4; the FP32 line-core pass 5 (as its one-spectrum kernel has
+// without a floor).
+constexpr int main_waves(int mode, int ppt, int lm) {
+    return ppt <= 4 ? (lm == 2 ? 3 : (mode == HK_VOIGT_FP32C ? 5 : 4)) : (ppt == 8 ? 3 : 1);
+}
+
 template <int MODE, int PPT, int LM>
-__global__ __launch_bounds__(256, PPT == 0 ? (MODE == HK_VOIGT_FP32C ? 5 : 1) : (PPT <= 4 ? (LM == 2 ? 3 : 4) : (PPT == 8 ? 3 : 1)))
+__global__ __launch_bounds__(256, PPT == 0 ? (MODE == HK_VOIGT_FP32C ? 5 : 1) : main_waves(MODE, PPT, LM))
 void voigt_persistent_kernel(const VArgs *__restrict__ specs,
                                                                const int64_t *__restrict__ item_start,
                                                                int nspec, unsigned long long *counter) {
