@@ -78,16 +78,17 @@ struct Rec64 {
 
 // FP32 record, grouped by the path that reads it: x and r2; the asymptotic
 // series of the wings with the strength folded into the coefficients
-// (far: B, B15, BC; mid: q2..q6, see far4_32); the 8-node rule and the
-// Doppler term; window and dispatch word.
+// (far: B, 1.5 B, BC; mid: q2..q6, see far4_32); the 8-node rule and the
+// Doppler term; window and dispatch word.  72 bytes: with the far field's
+// sums, a 256-line stage fits six workgroups in one CU's LDS.
 struct Rec32 {
     float r, vbh, vbl, y2;
-    float B, B15, BC;
+    float B, BC;
     int32_t d0;
     float q2, q3, q4, q5;
     float q6, G, A;
     int32_t dop;
-    int32_t code, lo, hi, pad;
+    int32_t code, lo, hi;
 };
 
 // Line-core pass records: lo/hi bound the points of the tile that lie both
@@ -374,7 +375,7 @@ constexpr double kMidMax32 = 1.0e300, kMid2Max32 = 3.0e4;
 //  ~59/r2^4 relative (< 1e-6 at r2 = 100).
 __device__ __forceinline__ float far32(const Rec32 &q, float r2) {
     const float inv = rcp32(r2);
-    return inv * fmaf(fmaf(q.BC, inv, q.B15), inv, q.B);
+    return inv * fmaf(fmaf(q.BC, inv, 1.5f * q.B), inv, q.B);
 }
 
 __device__ __forceinline__ float far4_32(const Rec32 &q, float r2) {
@@ -383,7 +384,7 @@ __device__ __forceinline__ float far4_32(const Rec32 &q, float r2) {
     p = fmaf(p, inv, q.q4);
     p = fmaf(p, inv, q.q3);
     p = fmaf(p, inv, q.q2);
-    p = fmaf(p, inv, q.B15);
+    p = fmaf(p, inv, 1.5f * q.B);
     p = fmaf(p, inv, q.B);
     return inv * p;
 }
@@ -403,11 +404,13 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
             const float x = fmaf(m, vbh, r0);
             v = PATH == kPathFar ? far32(qr, fmaf(x, x, y2)) : far4_32(qr, fmaf(x, x, y2));
         } else {
+            // these paths come with y^2 < kRF32 (else x^2 + y^2 >= kRF32 at
+            // every point, the far path), where the four-term series holds
+            // for every r2 >= kRM32: one wing formula, one branch fewer to
+            // diverge on
             const float x = fmaf(m, vbh, fmaf(m, vbl, r0));
             const float r2 = fmaf(x, x, y2);
-            if (r2 >= kRF32) {
-                v = far32(qr, r2);
-            } else if (r2 >= kRM32) {
+            if (r2 >= kRM32) {
                 v = far4_32(qr, r2);
             } else if (PATH == kPathMid2 || r2 >= kRB32) {
                 v = G * gh8_sum32(x * x, r2);
@@ -537,8 +540,20 @@ struct FarBand {
     float lo[N][kWaves];
     float xhi[N][kWaves][kWaves];     // [node][target][wave]
     float xlo[N][kWaves][kWaves];
-    double F[kWaves][N];              // node totals per target wave
-    double c[kWaves][N];              // Chebyshev coefficients per target wave
+};
+
+// Node totals and Chebyshev coefficients per target wave.  They exist only
+// after the line loop, so they live in the record array's LDS (FarCoefs).
+template <int N>
+struct FarCoef {
+    double F[kWaves][N];
+    double c[kWaves][N];
+};
+
+struct FarCoefs {
+    FarCoef<kFN1> b1;
+    FarCoef<kFN2> b2;
+    FarCoef<kFN3> b3;
 };
 
 struct FarLDS {
@@ -740,7 +755,7 @@ __device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, 
 // -- runs in FP64: a band-3 tile's far field varies by up to 25x across it,
 // and an FP32 Clenshaw sum would carry up to ~3e-5 of the smallest value.
 template <int N>
-__device__ __forceinline__ void band_sum(FarBand<N> *b, int t) {
+__device__ __forceinline__ void band_sum(const FarBand<N> *b, FarCoef<N> *fc, int t) {
     if (t >= 0 && t < N * kWaves) {
         const int tw = t / N, k = t % N;
         double F = 0.0;
@@ -748,12 +763,12 @@ __device__ __forceinline__ void band_sum(FarBand<N> *b, int t) {
         for (int w = 0; w < kWaves; ++w) F += (double)b->hi[k][w] + (double)b->lo[k][w];
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) F += (double)b->xhi[k][tw][w] + (double)b->xlo[k][tw][w];
-        b->F[tw][k] = F;
+        fc->F[tw][k] = F;
     }
 }
 
 template <int N>
-__device__ __forceinline__ void band_coeffs(FarBand<N> *b, int t) {
+__device__ __forceinline__ void band_coeffs(FarCoef<N> *b, int t) {
     if (t >= 0 && t < N * kWaves) {
         const int tw = t / N, m = t % N;
         // c_m = (2 - [m == 0]) / N sum_k F_k T_m(u_k), T_m by its recurrence
@@ -884,7 +899,6 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                 const double B = A * y * kISqrtPi, G = A * y * (2.0 / kPi);
                 const double yy = (double)r.y2;
                 r.B = (float)B;
-                r.B15 = (float)(1.5 * B);
                 // -2 y^2 B may exceed FP32 only for y > ~1e6, where the term
                 // it multiplies (-2 y^2 / r2^2 <= 2 / r2) is negligible
                 r.BC = (float)fmax(-2.0 * yy * B, -1.0e37);
@@ -944,17 +958,21 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         }
     }
     // far field: node sums -> Chebyshev coefficients -> the tile's points
+    // (totals and coefficients in the records' LDS, free after this barrier;
+    // the next tile writes records only after its own barriers)
+    static_assert(sizeof(FarCoefs) <= sizeof(Rec32) * kThreads, "far coefficients overlay");
+    FarCoefs *fc = reinterpret_cast<FarCoefs *>(rec);
     __syncthreads();
     const int any = fl->any1 | (fl->any2 << 1) | (fl->any3 << 2);
     if (any) {
         // band 1 on threads 0..35, band 2 on 40..99, band 3 on 100..175
-        band_sum<kFN1>(&fl->b1, t);
-        band_sum<kFN2>(&fl->b2, t - 40);
-        band_sum<kFN3>(&fl->b3, t - 100);
+        band_sum<kFN1>(&fl->b1, &fc->b1, t);
+        band_sum<kFN2>(&fl->b2, &fc->b2, t - 40);
+        band_sum<kFN3>(&fl->b3, &fc->b3, t - 100);
         __syncthreads();
-        band_coeffs<kFN1>(&fl->b1, t);
-        band_coeffs<kFN2>(&fl->b2, t - 40);
-        band_coeffs<kFN3>(&fl->b3, t - 100);
+        band_coeffs<kFN1>(&fc->b1, t);
+        band_coeffs<kFN2>(&fc->b2, t - 40);
+        band_coeffs<kFN3>(&fc->b3, t - 100);
         __syncthreads();
     }
 #pragma unroll
@@ -965,9 +983,9 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         if (any) {
             // the interpolants at u = (jr - hW) / hW
             const double u = ((double)(jr0 + 64 * k) - hW) / hW;
-            if (any & 1) ff += cheb_eval<kFN1>(fl->b1.c[wave], u);
-            if (any & 2) ff += cheb_eval<kFN2>(fl->b2.c[wave], u);
-            if (any & 4) ff += cheb_eval<kFN3>(fl->b3.c[wave], u);
+            if (any & 1) ff += cheb_eval<kFN1>(fc->b1.c[wave], u);
+            if (any & 2) ff += cheb_eval<kFN2>(fc->b2.c[wave], u);
+            if (any & 4) ff += cheb_eval<kFN3>(fc->b3.c[wave], u);
         }
         if (j < a.nx) {
             const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
@@ -1164,16 +1182,16 @@ __global__ __launch_bounds__(256) void voigt_core_kernel(VArgs a) {
 // selects the line-core pass.
 // Occupancy floors (waves per SIMD): the FP32C main pass is latency-bound
 // (LDS record reads, staging loads, a barrier per stage), so occupancy beats
-// registers: 5 for tiles up to 1024 points with 256-line stages (LDS allows
-// 5 workgroups per CU; the few values that then spill are reloaded once per
-// stage or per tile, outside the record loops: c2's batched sum 1070 -> 964
-// ms against 4), 3 with 512-line stages (LDS allows 3 workgroups per CU) and
+// registers: 6 for tiles up to 1024 points with 256-line stages (25 KB of
+// LDS per workgroup; the values that then spill are reloaded once per stage
+// or per tile, outside the record loops: c2's batched sum 1070 ms at 4, 964
+// at 5, 891 at 6), 3 with 512-line stages (LDS allows 3 workgroups per CU) and
 // for 2048-point tiles (without a floor the far-field code raises the
 // register count of the persistent form enough to halve its occupancy); the
 // FP64 main pass 4; the FP32 line-core pass 5 (as its one-spectrum kernel has
 // without a floor).
 constexpr int main_waves(int mode, int ppt, int lm) {
-    return ppt <= 4 ? (lm == 2 ? 3 : (mode == HK_VOIGT_FP32C ? 5 : 4)) : (ppt == 8 ? 3 : 1);
+    return ppt <= 4 ? (lm == 2 ? 3 : (mode == HK_VOIGT_FP32C ? 6 : 4)) : (ppt == 8 ? 3 : 1);
 }
 
 template <int MODE, int PPT, int LM>
