@@ -538,8 +538,11 @@ template <int N>
 struct FarBand {
     float hi[N][kWaves];              // common node sums, one column per wave
     float lo[N][kWaves];
-    float xhi[N][kWaves][kWaves];     // [node][target][wave]
-    float xlo[N][kWaves][kWaves];
+    // [node][target][wave]; band 1 only: a line whose window ends inside the
+    // tile lies a cut from its centre, far from the tile, and the other
+    // bands' lines with such windows stay with the pair-by-pair path
+    float xhi[N == kFN1 ? N : 1][kWaves][kWaves];
+    float xlo[N == kFN1 ? N : 1][kWaves][kWaves];
 };
 
 // Node totals and Chebyshev coefficients per target wave.  They exist only
@@ -550,16 +553,23 @@ struct FarCoef {
     double c[kWaves][N];
 };
 
+// Wave-level far field (wave_far_codes): per wave, node sums of its three
+// bands one after the other (9, 15, 19 nodes) and their coefficients.
+constexpr int kWN = kFN1 + kFN2 + kFN3;
+constexpr int kWOff2 = kFN1, kWOff3 = kFN1 + kFN2;
+
 struct FarCoefs {
     FarCoef<kFN1> b1;
     FarCoef<kFN2> b2;
     FarCoef<kFN3> b3;
+    double wc[kWaves][kWN];
 };
 
 struct FarLDS {
     FarBand<kFN1> b1;
     FarBand<kFN2> b2;
     FarBand<kFN3> b3;
+    float ws[kWaves][kWN][2];         // wave-level node sums (hi, lo)
     int cnt[2][kWaves];               // compaction counts, alternate stages
     int any1, any2, any3;             // band used (set to 1 by any wave)
 };
@@ -682,15 +692,51 @@ __device__ __forceinline__ int far_cover(int32_t &lo, int32_t &hi, int Wv, int w
     return mask;
 }
 
+// The tile bands' distance rules for an interval of half width `half`
+// whose centre lies dc from the line centre, both in Doppler units, for
+// the wave-level far field: also x^2 + y^2 >= kRF32 at the interval's
+// nearest point, so the node values take the two-term series of far32.
+__device__ __forceinline__ int wave_band(double dc, double half, double y2) {
+    if (dc < kFarRatio3 * half) return 0;
+    const double xn = dc - half;
+    if (!(fma(xn, xn, y2) >= (double)kRF32)) return 0;
+    return dc >= kFarRatio1 * half ? 1 : (dc >= kFarRatio2 * half ? 2 : 3);
+}
+
+// Wave-level far field.  A line that the tile takes pair by pair (its centre
+// too near the tile for the tile's far field) can lie far from some of the
+// tile's waves: wave w (tile-relative points [w wpts, (w + 1) wpts), all
+// valid and all inside the line's window) takes the line from a far field of
+// its own -- Chebyshev nodes over the wave's interval, the tile bands' rules
+// with the wave's half width -- instead of pair by pair.  x at tile-relative
+// point j is (j + d0) vb + rr.  Returns per wave (4 bits each) 8 + band, or 0.
+template <int WPTS>
+__device__ __forceinline__ int32_t wave_far_codes(int32_t lo, int32_t hi, int64_t Wv, double d0,
+                                                  double vb, double rr, double y2) {
+    constexpr double hwp = 0.5 * (double)(WPTS - 1);
+    const double half = hwp * vb;
+    int32_t code = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int a = w * WPTS, b = a + WPTS;
+        if ((int64_t)b > Wv || lo > a || hi < b) continue;
+        const int band = wave_band(fabs(((double)a + hwp + d0) * vb + rr), half, y2);
+        if (band) code |= (8 + band) << (4 * w);
+    }
+    return code;
+}
+
 template <int N>
 __device__ __forceinline__ void band_init(FarBand<N> *b, int t) {
     for (int e = t; e < N * kWaves; e += kThreads) {
         (&b->hi[0][0])[e] = 0.0f;
         (&b->lo[0][0])[e] = 0.0f;
     }
-    for (int e = t; e < N * kWaves * kWaves; e += kThreads) {
-        (&b->xhi[0][0][0])[e] = 0.0f;
-        (&b->xlo[0][0][0])[e] = 0.0f;
+    if constexpr (N == kFN1) {
+        for (int e = t; e < N * kWaves * kWaves; e += kThreads) {
+            (&b->xhi[0][0][0])[e] = 0.0f;
+            (&b->xlo[0][0][0])[e] = 0.0f;
+        }
     }
 }
 
@@ -726,7 +772,7 @@ __device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, 
         } else {
             v = mine ? fB * far_node32(x, fy) : 0.0f;
         }
-        if (!split) {
+        if (N != kFN1 || !split) {
             const float sw = wave_total32(v);
             if (lane == 63) {
                 float hi = b->hi[k][wave], lo = b->lo[k][wave];
@@ -761,8 +807,10 @@ __device__ __forceinline__ void band_sum(const FarBand<N> *b, FarCoef<N> *fc, in
         double F = 0.0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) F += (double)b->hi[k][w] + (double)b->lo[k][w];
+        if constexpr (N == kFN1) {
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) F += (double)b->xhi[k][tw][w] + (double)b->xlo[k][tw][w];
+            for (int w = 0; w < kWaves; ++w) F += (double)b->xhi[k][tw][w] + (double)b->xlo[k][tw][w];
+        }
         fc->F[tw][k] = F;
     }
 }
@@ -803,6 +851,51 @@ __device__ __forceinline__ double cheb_eval(const double *c, double u) {
     return fma(u, b1, c[0] - b2);
 }
 
+// Wave-level far field, node sums: lane l holds the record of its own line
+// (record c0 + l of the stage); the lanes whose line this wave takes in
+// band N (mine) add its values at the N nodes of the wave's interval
+// (centre cw, half width hwp in points) to the wave's compensated sums s.
+// x = m vb + rr in FP32 as in the pair-by-pair path, m = cw + d0 + u_k hwp
+// (|x| >= |D| / 3 for these lines); the value is far32's two-term series.
+template <int N>
+__device__ __forceinline__ void wave_far_nodes(const Rec32 &q, bool mine, float cw, float hwp,
+                                               float (*s)[2], int lane) {
+    const float m0 = cw + (float)q.d0;
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        const float m = fmaf((float)far_u<N>(k), hwp, m0);
+        const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
+        const float v = mine ? far32(q, fmaf(x, x, q.y2)) : 0.0f;
+        const float sw = wave_total32(v);
+        if (lane == 63) {
+            float h = s[k][0], l = s[k][1];
+            two_sum_acc(h, l, sw);
+            s[k][0] = h;
+            s[k][1] = l;
+        }
+    }
+}
+
+// Wave-level far field, coefficients of one band from the wave's node sums
+// (FP64, as the tile bands), by lane m < N.
+template <int N>
+__device__ __forceinline__ void wave_far_coeffs(const float (*s)[2], double *c, int m) {
+    if (m < 0 || m >= N) return;
+    double acc = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        const double u = far_u<N>(k);
+        double tm = 1.0, tc = u;
+        for (int q = 2; q <= m; ++q) {
+            const double tn = fma(2.0 * u, tc, -tm);
+            tm = tc;
+            tc = tn;
+        }
+        acc += ((double)s[k][0] + (double)s[k][1]) * (m == 0 ? 1.0 : tc);
+    }
+    c[m] = (m == 0 ? 1.0 : 2.0) / (double)N * acc;
+}
+
 template <int PPT, int LM>
 __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *rec, FarLDS *fl) {
     constexpr int W = kThreads * PPT;
@@ -824,6 +917,12 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
     band_init<kFN1>(&fl->b1, t);
     band_init<kFN2>(&fl->b2, t);
     band_init<kFN3>(&fl->b3, t);
+    // wave-level far field: this wave's node sums and the bands it used
+    constexpr float whp = 0.5f * (float)(64 * PPT - 1);
+    const float wcen = (float)wlo + whp;
+    float (*ws)[2] = fl->ws[wave];
+    for (int e = lane; e < 2 * kWN; e += 64) (&ws[0][0])[e] = 0.0f;
+    int wused = 0;
     const double xc = fma((double)j0 + hW, a.dnu, a.numin);     // tile centre [cm^-1]
     if (t == 0) { fl->any1 = 0; fl->any2 = 0; fl->any3 = 0; }
     __syncthreads();
@@ -856,7 +955,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                     clo = (int32_t)(max(wl, j0) - j0);
                     chi = (int32_t)(min(wh, j0 + W) - j0);
                     if (band) {
-                        cover = clo < chi ? far_cover(clo, chi, (int)Wv, 64 * PPT) : 0;
+                        cover = band == 1 && clo < chi ? far_cover(clo, chi, (int)Wv, 64 * PPT) : 0;
                         if (!cover) band = 0;
                     }
                 }
@@ -926,6 +1025,11 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                     return fmax(xa * xa, xb * xb) + y2;
                 };
                 r.code = wave_codes<64 * PPT>(r.lo, r.hi, ri, r2max, kMidMax32, kMid2Max32);
+                if (ffok) {
+                    // waves far from the line take it from their own far field
+                    const int32_t wf = wave_far_codes<64 * PPT>(r.lo, r.hi, Wv, d0, vb, rr, y2);
+                    r.code = (r.code & ~(((wf & 0x88888888) >> 3) * 0xF)) | wf;
+                }
             }
         }
         // Barriers: the previous stage's readers of rec[] must be done before
@@ -955,6 +1059,14 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             run_path<kPathMid2, false, PPT>(rec, c0, __ballot(nib == 6), part, hi, lo, since, jr0);
             run_path<kPathAny, true, PPT>(rec, c0, __ballot(nib == 7), part, hi, lo, since, jr0);
             run_path<kPathAny, false, PPT>(rec, c0, __ballot(nib == 8), part, hi, lo, since, jr0);
+            // wave-level far field: each lane evaluates its own record's line
+            const unsigned long long wfm = __ballot(nib > 8);
+            if (wfm) {
+                const Rec32 q = rec[min(c0 + lane, nst - 1)];
+                if (__ballot(nib == 9)) { wave_far_nodes<kFN1>(q, nib == 9, wcen, whp, ws, lane); wused |= 1; }
+                if (__ballot(nib == 10)) { wave_far_nodes<kFN2>(q, nib == 10, wcen, whp, ws + kWOff2, lane); wused |= 2; }
+                if (__ballot(nib == 11)) { wave_far_nodes<kFN3>(q, nib == 11, wcen, whp, ws + kWOff3, lane); wused |= 4; }
+            }
         }
     }
     // far field: node sums -> Chebyshev coefficients -> the tile's points
@@ -973,8 +1085,16 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         band_coeffs<kFN1>(&fc->b1, t);
         band_coeffs<kFN2>(&fc->b2, t - 40);
         band_coeffs<kFN3>(&fc->b3, t - 100);
-        __syncthreads();
     }
+    // wave-level coefficients, each wave its own (lanes 0..42), read by the
+    // wave's lanes after the barrier
+    if (ffok) {
+        double *wc = fc->wc[wave];
+        if (wused & 1) wave_far_coeffs<kFN1>(ws, wc, lane);
+        if (wused & 2) wave_far_coeffs<kFN2>(ws + kWOff2, wc + kWOff2, lane - kWOff2);
+        if (wused & 4) wave_far_coeffs<kFN3>(ws + kWOff3, wc + kWOff3, lane - kWOff3);
+    }
+    if (any || ffok) __syncthreads();
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         two_sum_acc(hi[k], lo[k], part[k]);
@@ -986,6 +1106,14 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             if (any & 1) ff += cheb_eval<kFN1>(fc->b1.c[wave], u);
             if (any & 2) ff += cheb_eval<kFN2>(fc->b2.c[wave], u);
             if (any & 4) ff += cheb_eval<kFN3>(fc->b3.c[wave], u);
+        }
+        if (wused) {
+            // the wave's own interpolants, u over the wave's interval
+            const double u = ((double)(jr0 + 64 * k) - (double)wcen) / (double)whp;
+            const double *wc = fc->wc[wave];
+            if (wused & 1) ff += cheb_eval<kFN1>(wc, u);
+            if (wused & 2) ff += cheb_eval<kFN2>(wc + kWOff2, u);
+            if (wused & 4) ff += cheb_eval<kFN3>(wc + kWOff3, u);
         }
         if (j < a.nx) {
             const double sum = ldexp(((double)hi[k] + (double)lo[k]) + ff, ti.e);
@@ -1296,12 +1424,13 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                                                          double numin, double dnu, int64_t nx,
                                                          int ppt, int far,
                                                          unsigned long long *__restrict__ out) {
-    constexpr int kOut = 7;
+    constexpr int kOut = 10;
     __shared__ unsigned long long red[kOut][256];
     const int t = threadIdx.x;
     const int W = kThreads * ppt, WP = 64 * ppt;
     const int64_t i = (int64_t)blockIdx.x * 256 + t;
-    // direct A, B, C; far-field pairs; far (line, tile) pairs of bands 1-3
+    // direct A, B, C; far-field pairs; far (line, tile) pairs of bands 1-3;
+    // far (line, wave) pairs of the wave-level bands 1-3
     unsigned long long c[kOut] = {};
     if (i < nl) {
         int64_t wl, wh, bl = 0, bh = 0, cl = 0, ch = 0;
@@ -1322,19 +1451,42 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                 // the pairs left to the pair-by-pair path: [dlo, dhi)
                 int32_t dlo = (int32_t)(max(wl, j0) - j0), dhi = (int32_t)(min(wh, j0 + Wv) - j0);
                 const int32_t n0 = dhi - dlo;
-                if (band && !full && !far_cover(dlo, dhi, (int)Wv, WP)) band = 0;
+                if (band && !full && (band != 1 || !far_cover(dlo, dhi, (int)Wv, WP))) band = 0;
                 if (band) {
                     c[3] += (unsigned long long)(full ? Wv : n0 - (dhi - dlo));
                     c[3 + band] += 1;
                     if (full) continue;
                 }
-                const int64_t p0 = j0 + dlo, p1 = j0 + dhi;
-                const unsigned long long tot = ov(wl, wh, p0, p1);
-                const unsigned long long bc = ov(bl, bh, p0, p1);
-                const unsigned long long cc = ov(cl, ch, p0, p1);
-                c[0] += tot - bc;
-                c[1] += bc - cc;
-                c[2] += cc;
+                // waves that take the line from their own far field (the
+                // kernel's test, on the record's FP32 y)
+                int32_t wf = 0;
+                if (far && Wv >= kFarMinPts && dlo < dhi) {
+                    double rr, vb, d0;
+                    line_offsets(nu[i], a, numin, dnu, j0, rr, vb, d0);
+                    const double yf = (double)(float)yy, y2f = yf * yf;
+                    switch (ppt) {
+                        case 2: wf = wave_far_codes<128>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
+                        case 4: wf = wave_far_codes<256>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
+                        case 8: wf = wave_far_codes<512>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
+                        default: wf = wave_far_codes<1024>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
+                    }
+                }
+                for (int w = 0; w < kWaves; ++w) {
+                    const int64_t p0 = j0 + max(dlo, w * WP), p1 = j0 + min(dhi, (w + 1) * WP);
+                    if (p1 <= p0) continue;
+                    const int wb = (wf >> (4 * w)) & 15;
+                    if (wb) {
+                        c[3] += (unsigned long long)(p1 - p0);
+                        c[7 + wb - 9] += 1;
+                        continue;
+                    }
+                    const unsigned long long tot = ov(wl, wh, p0, p1);
+                    const unsigned long long bc = ov(bl, bh, p0, p1);
+                    const unsigned long long cc = ov(cl, ch, p0, p1);
+                    c[0] += tot - bc;
+                    c[1] += bc - cc;
+                    c[2] += cc;
+                }
             }
         }
     }
@@ -1348,8 +1500,8 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
     if (t < kOut) atomicAdd(&out[t], red[t][0]);
 }
 
-__global__ void zero8_kernel(unsigned long long *c) {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0ull;
+__global__ void zero12_kernel(unsigned long long *c) {
+    if (threadIdx.x < 12) c[threadIdx.x] = 0ull;
 }
 
 }  // namespace hk
@@ -1839,7 +1991,7 @@ extern "C" int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid,
     HK_CHECK_ARG(ppt_ok(ppt), "hk_voigt_work_count: tile_points must be 512, 1024, 2048 or 4096");
     const int far = (mode == HK_VOIGT_FP32C && !(tune && tune->exact_pairs)) ? 1 : 0;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(zero8_kernel, dim3(1), dim3(64), 0, s, counts);
+    hipLaunchKernelGGL(zero12_kernel, dim3(1), dim3(64), 0, s, counts);
     HK_LAUNCH_CHECK();
     if (st->nlines > 0) {
         hipLaunchKernelGGL(work_count_kernel, dim3((unsigned)((st->nlines + 255) / 256)), dim3(256),

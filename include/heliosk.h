@@ -144,8 +144,9 @@ typedef struct {
     int tile_points;    /* grid points per workgroup tile: 512, 1024
                            (default), 2048 or 4096 */
     int exact_pairs;    /* HK_VOIGT_FP32C: 1 evaluates every (line, point)
-                           pair; 0 (default) sums lines far from a tile
-                           through its Chebyshev far field */
+                           pair; 0 (default) sums lines far from a tile,
+                           or from a wave's quarter of it, through
+                           Chebyshev far fields */
     int stage_lines;    /* lines staged through LDS per step of the main
                            pass: 256 (default) or 512 (the reference's fixed
                            line blocks def_nlA/B/C, define.h:56-58); FP64
@@ -221,14 +222,15 @@ int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
                         unsigned long long *counts, hk_stream_t stream);
 
 /* Work actually done by one hk_voigt_accumulate call with this mode and
- * tuning (tile width set by tile_points): counts (8 device uint64) =
+ * tuning (tile width set by tile_points): counts (12 device uint64) =
  * [direct pairs in regions A, B, C (as hk_voigt_pair_count), pairs summed
- * through the FP32 far field, far (line, tile) pairs of band 1 -- each 9
+ * through the FP32 far fields, far (line, tile) pairs of band 1 -- each 9
  * node evaluations --, of band 2 -- 15 each --, of band 3 -- 19 each --,
- * 0].  A far line whose window ends inside a tile counts as a far (line,
- * tile) pair; its pairs in the wave that holds the window's end are direct.
- * The direct and far-field pairs add up to counts[0] of
- * hk_voigt_pair_count. */
+ * far (line, wave) pairs of the wave-level far field in bands 1, 2, 3 (the
+ * same node counts), 0, 0].  A far line whose window ends inside a tile
+ * counts as a far (line, tile) pair; its pairs in the wave that holds the
+ * window's end are direct.  The direct and far-field pairs add up to
+ * counts[0] of hk_voigt_pair_count. */
 int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
                         const hk_tune *tune, unsigned long long *counts, hk_stream_t stream);
 
