@@ -8,8 +8,9 @@
 //     centres, the lines whose cut window can reach the tile, and (FP32
 //     form) a power-of-two strength scale for the tile;
 //   * each workgroup stages 256 lines at a time through LDS; every thread
-//     owns W/256 grid points and keeps their sums in registers, adding the
-//     lines in index order -- no atomics, bit-identical reruns;
+//     owns W/256 grid points and keeps their sums in registers (the FP64
+//     form adds the lines in index order, the FP32 form path by path within
+//     a stage) -- no atomics, bit-identical reruns;
 //   * a line contributes to point j iff |x_j - nu| < hw in FP64 (the
 //     integer window is computed once per line and tile, cut_window).
 //
@@ -35,8 +36,10 @@
 //                  asymptotic); per-point FP32 partial sums over 32 lines
 //                  folded into a two-sum (hi, lo) pair; strengths scaled by
 //                  2^-e per tile so the largest maps to 2^60.
-//                  Far lines (see tile_f32c) are summed at 11 Chebyshev
-//                  nodes of the tile in FP64 and interpolated to its points.
+//                  Far lines (see the far-field section) are summed in FP32
+//                  at the Chebyshev nodes of the tile (9, 15 or 19 by
+//                  distance), or of a wave's quarter of it, and interpolated
+//                  to the points in FP64.
 #include <math.h>
 
 #include <algorithm>
