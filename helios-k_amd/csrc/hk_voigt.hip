@@ -763,6 +763,9 @@ __device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, 
     const bool series2 = __ballot(mine && !(fma(xn, xn, (double)fy * fy) >= (double)kRF32)) == 0ull;
     const float y2 = fy * fy, B = fB * fy * (float)(1.0 / kPi);
     const float B15 = 1.5f * B, BC = fmaxf(-2.0f * y2 * B, -1.0e37f);
+    // the wave totals are gathered into lanes (node k, or target tw and node
+    // k, in lane tw N + k) and added to the columns in one pass at the end
+    float tot = 0.0f;
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
         // far lines: |x| >= |D| - h >= |D| / 3, so x in FP32 from FP32 D and
@@ -776,25 +779,31 @@ __device__ __forceinline__ void band_nodes(FarBand<N> *b, bool mine, int cover, 
             v = mine ? fB * far_node32(x, fy) : 0.0f;
         }
         if (N != kFN1 || !split) {
-            const float sw = wave_total32(v);
-            if (lane == 63) {
-                float hi = b->hi[k][wave], lo = b->lo[k][wave];
-                two_sum_acc(hi, lo, sw);
-                b->hi[k][wave] = hi;
-                b->lo[k][wave] = lo;
-            }
+            const float sw = __int_as_float(
+                __builtin_amdgcn_readlane(__float_as_int(wave_total32(v)), 63));
+            tot = lane == k ? sw : tot;
         } else {
 #pragma unroll
             for (int tw = 0; tw < kWaves; ++tw) {
-                const float sw = wave_total32((cover >> tw) & 1 ? v : 0.0f);
-                if (lane == 63) {
-                    float hi = b->xhi[k][tw][wave], lo = b->xlo[k][tw][wave];
-                    two_sum_acc(hi, lo, sw);
-                    b->xhi[k][tw][wave] = hi;
-                    b->xlo[k][tw][wave] = lo;
-                }
+                const float sw = __int_as_float(__builtin_amdgcn_readlane(
+                    __float_as_int(wave_total32((cover >> tw) & 1 ? v : 0.0f)), 63));
+                tot = lane == tw * N + k ? sw : tot;
             }
         }
+    }
+    if (N != kFN1 || !split) {
+        if (lane < N) {
+            float hi = b->hi[lane][wave], lo = b->lo[lane][wave];
+            two_sum_acc(hi, lo, tot);
+            b->hi[lane][wave] = hi;
+            b->lo[lane][wave] = lo;
+        }
+    } else if (lane < kWaves * N) {
+        const int tw = lane / N, k = lane - tw * N;
+        float hi = b->xhi[k][tw][wave], lo = b->xlo[k][tw][wave];
+        two_sum_acc(hi, lo, tot);
+        b->xhi[k][tw][wave] = hi;
+        b->xlo[k][tw][wave] = lo;
     }
 }
 
@@ -864,18 +873,21 @@ template <int N>
 __device__ __forceinline__ void wave_far_nodes(const Rec32 &q, bool mine, float cw, float hwp,
                                                float (*s)[2], int lane) {
     const float m0 = cw + (float)q.d0;
+    float tot = 0.0f;           // lane k: node k's wave total
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
         const float m = fmaf((float)far_u<N>(k), hwp, m0);
         const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
         const float v = mine ? far32(q, fmaf(x, x, q.y2)) : 0.0f;
-        const float sw = wave_total32(v);
-        if (lane == 63) {
-            float h = s[k][0], l = s[k][1];
-            two_sum_acc(h, l, sw);
-            s[k][0] = h;
-            s[k][1] = l;
-        }
+        const float sw = __int_as_float(
+            __builtin_amdgcn_readlane(__float_as_int(wave_total32(v)), 63));
+        tot = lane == k ? sw : tot;
+    }
+    if (lane < N) {
+        float h = s[lane][0], l = s[lane][1];
+        two_sum_acc(h, l, tot);
+        s[lane][0] = h;
+        s[lane][1] = l;
     }
 }
 
