@@ -631,6 +631,18 @@ __device__ __forceinline__ float wave_total32(float v) {
     return v;
 }
 
+// Maximum of v over the wave, complete in lane 63 (the DPP tree of
+// wave_total32).
+__device__ __forceinline__ int wave_max_i32(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
 // Far-field band (0: evaluated point by point) of a line (centre nu, cut
 // half width hw, 1/alpha_D ia, y) for the tile [j0, j0 + Wv) with half
 // width hW (points): its centre is at least kFarRatio3 half widths from the
@@ -1268,16 +1280,65 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
             }
         }
         __syncthreads();
-        for (int l = 0; l < n; ++l) {
-            const Core32 &q = rec[l];
-            if ((q.hi <= wlo) | (q.lo >= whi)) continue;      // one LDS read for both
+        // Records 64 at a time: each lane marks the records whose core
+        // interval holds its point (bit j: record c0 + j).  Where that
+        // leaves the busiest lane well under the records that reach the
+        // wave (narrow cores: a 64-point wave meets far more cores than
+        // cover any one of its points), the lanes take their records in
+        // ascending order, one per round, so a round evaluates a core for
+        // every lane that still has one; otherwise the wave walks the
+        // records with uniform loads.  Either way a point sees its records
+        // in line order.
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int cn = min(64, n - c0);
+            int rlo = 0, rhi = 0;
+            if (lane < cn) { rlo = rec[c0 + lane].lo; rhi = rec[c0 + lane].hi; }
+            const unsigned long long m0 = __ballot(lane < cn && rhi > wlo && rlo < whi);
+            if (!m0) continue;
             touched = true;
-            const float m = (float)(jr + q.d0);
-            const float x = fmaf(m, q.vbh, fmaf(m, q.vbl, q.r));
-            const float r2 = fmaf(x, x, q.y2);
-            if (jr >= q.lo && jr < q.hi && r2 < kRB32)
-                part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
-            if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+            // when one point of the wave lies in every record's interval
+            // (dense or wide cores), each record needs a round anyway
+            const bool in0 = (m0 >> lane) & 1ull;
+            const int maxlo = __builtin_amdgcn_readlane(wave_max_i32(in0 ? max(rlo, wlo) : 0), 63);
+            const int minhi = (1 << 20) - __builtin_amdgcn_readlane(
+                                  wave_max_i32(in0 ? (1 << 20) - min(rhi, whi) : 0), 63);
+            unsigned long long cov = 0ull, m = m0;
+            int rounds = __popcll(m0);
+            if (maxlo >= minhi) {
+                while (m) {
+                    const int b = lowest_bit(m);
+                    const int blo = __builtin_amdgcn_readlane(rlo, b);
+                    const int bhi = __builtin_amdgcn_readlane(rhi, b);
+                    if (jr >= blo && jr < bhi) cov |= 1ull << b;
+                }
+                rounds = __builtin_amdgcn_readlane(wave_max_i32(__popcll(cov)), 63);
+            }
+            if (4 * rounds >= 3 * __popcll(m0)) {
+                m = m0;
+                while (m) {
+                    const int l = c0 + lowest_bit(m);
+                    const Core32 &q = rec[l];
+                    const float mm = (float)(jr + q.d0);
+                    const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
+                    const float r2 = fmaf(x, x, q.y2);
+                    if (jr >= q.lo && jr < q.hi && r2 < kRB32)
+                        part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
+                    if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+                }
+                continue;
+            }
+            while (__ballot(cov != 0ull)) {
+                if (cov) {
+                    const int l = c0 + __builtin_ctzll(cov);
+                    cov &= cov - 1ull;
+                    const Core32 &q = rec[l];
+                    const float mm = (float)(jr + q.d0);
+                    const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
+                    const float r2 = fmaf(x, x, q.y2);
+                    if (r2 < kRB32) part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
+                }
+                if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+            }
         }
     }
     two_sum_acc(hi, lo, part);
