@@ -1280,65 +1280,84 @@ __device__ __forceinline__ void core_f32c(const VArgs &a, int64_t tile, Core32 *
             }
         }
         __syncthreads();
-        // Records 64 at a time: each lane marks the records whose core
-        // interval holds its point (bit j: record c0 + j).  Where that
-        // leaves the busiest lane well under the records that reach the
-        // wave (narrow cores: a 64-point wave meets far more cores than
-        // cover any one of its points), the lanes take their records in
-        // ascending order, one per round, so a round evaluates a core for
-        // every lane that still has one; otherwise the wave walks the
-        // records with uniform loads.  Either way a point sees its records
-        // in line order.
-        for (int c0 = 0; c0 < n; c0 += 64) {
-            const int cn = min(64, n - c0);
-            int rlo = 0, rhi = 0;
-            if (lane < cn) { rlo = rec[c0 + lane].lo; rhi = rec[c0 + lane].hi; }
-            const unsigned long long m0 = __ballot(lane < cn && rhi > wlo && rlo < whi);
-            if (!m0) continue;
-            touched = true;
-            // when one point of the wave lies in every record's interval
-            // (dense or wide cores), each record needs a round anyway
-            const bool in0 = (m0 >> lane) & 1ull;
-            const int maxlo = __builtin_amdgcn_readlane(wave_max_i32(in0 ? max(rlo, wlo) : 0), 63);
-            const int minhi = (1 << 20) - __builtin_amdgcn_readlane(
-                                  wave_max_i32(in0 ? (1 << 20) - min(rhi, whi) : 0), 63);
-            unsigned long long cov = 0ull, m = m0;
-            int rounds = __popcll(m0);
-            if (maxlo >= minhi) {
+        // The stage's records (up to kStage, in four words of 64) that
+        // reach this wave.  Unless one of its points lies in every such
+        // record's interval (then each record needs a round anyway: wide
+        // cores, or a stage of dense narrow ones), each lane marks the
+        // records whose core interval holds its own point, and if the
+        // busiest lane then holds under three quarters of them, the lanes
+        // take their records in ascending order, one per round, so every
+        // round evaluates a core for each lane that still has one (a
+        // 64-point wave meets far more narrow cores than cover any one of
+        // its points).  Otherwise the wave walks the records with uniform
+        // loads.  A point sees its records in line order either way.
+        static_assert(kStage == 256, "four mask words");
+        unsigned long long mw[4], cov[4];
+        int rlo[4], rhi[4], nrec = 0, maxlo = 0, minhi = 1 << 20;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int c0 = 64 * w, cn = min(64, n - c0);
+            rlo[w] = 0;
+            rhi[w] = 0;
+            if (lane < cn) { rlo[w] = rec[c0 + lane].lo; rhi[w] = rec[c0 + lane].hi; }
+            mw[w] = __ballot(lane < cn && rhi[w] > wlo && rlo[w] < whi);
+            cov[w] = 0ull;
+            nrec += __popcll(mw[w]);
+            const bool in = (mw[w] >> lane) & 1ull;
+            maxlo = max(maxlo, in ? max(rlo[w], wlo) : 0);
+            minhi = min(minhi, in ? min(rhi[w], whi) : (1 << 20));
+        }
+        if (!nrec) continue;
+        touched = true;
+        maxlo = __builtin_amdgcn_readlane(wave_max_i32(maxlo), 63);
+        minhi = (1 << 20) - __builtin_amdgcn_readlane(wave_max_i32((1 << 20) - minhi), 63);
+        int rounds = nrec;
+        if (maxlo >= minhi) {
+            int mine = 0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                unsigned long long m = mw[w];
                 while (m) {
                     const int b = lowest_bit(m);
-                    const int blo = __builtin_amdgcn_readlane(rlo, b);
-                    const int bhi = __builtin_amdgcn_readlane(rhi, b);
-                    if (jr >= blo && jr < bhi) cov |= 1ull << b;
+                    const int blo = __builtin_amdgcn_readlane(rlo[w], b);
+                    const int bhi = __builtin_amdgcn_readlane(rhi[w], b);
+                    if (jr >= blo && jr < bhi) cov[w] |= 1ull << b;
                 }
-                rounds = __builtin_amdgcn_readlane(wave_max_i32(__popcll(cov)), 63);
+                mine += __popcll(cov[w]);
             }
-            if (4 * rounds >= 3 * __popcll(m0)) {
-                m = m0;
-                while (m) {
-                    const int l = c0 + lowest_bit(m);
-                    const Core32 &q = rec[l];
-                    const float mm = (float)(jr + q.d0);
-                    const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
-                    const float r2 = fmaf(x, x, q.y2);
-                    if (jr >= q.lo && jr < q.hi && r2 < kRB32)
-                        part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
-                    if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
-                }
-                continue;
+            rounds = __builtin_amdgcn_readlane(wave_max_i32(mine), 63);
+        }
+        auto eval_core = [&](int l) {
+            const Core32 &q = rec[l];
+            const float mm = (float)(jr + q.d0);
+            const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
+            const float r2 = fmaf(x, x, q.y2);
+            if (jr >= q.lo && jr < q.hi && r2 < kRB32)
+                part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
+            if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+        };
+        if (4 * rounds >= 3 * nrec) {
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                unsigned long long m = mw[w];
+                while (m) eval_core(64 * w + lowest_bit(m));
             }
-            while (__ballot(cov != 0ull)) {
-                if (cov) {
-                    const int l = c0 + __builtin_ctzll(cov);
-                    cov &= cov - 1ull;
-                    const Core32 &q = rec[l];
-                    const float mm = (float)(jr + q.d0);
-                    const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
-                    const float r2 = fmaf(x, x, q.y2);
-                    if (r2 < kRB32) part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
-                }
-                if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
+            continue;
+        }
+        while (__ballot((cov[0] | cov[1] | cov[2] | cov[3]) != 0ull)) {
+            int l = -1;
+            if (cov[0]) { l = __builtin_ctzll(cov[0]); cov[0] &= cov[0] - 1ull; }
+            else if (cov[1]) { l = 64 + __builtin_ctzll(cov[1]); cov[1] &= cov[1] - 1ull; }
+            else if (cov[2]) { l = 128 + __builtin_ctzll(cov[2]); cov[2] &= cov[2] - 1ull; }
+            else if (cov[3]) { l = 192 + __builtin_ctzll(cov[3]); cov[3] &= cov[3] - 1ull; }
+            if (l >= 0) {
+                const Core32 &q = rec[l];
+                const float mm = (float)(jr + q.d0);
+                const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
+                const float r2 = fmaf(x, x, q.y2);
+                if (r2 < kRB32) part += q.A * rew_core32(x, q.y, q.ey, q.q1, dtab + l * kDT);
             }
+            if (++since == 32) { two_sum_acc(hi, lo, part); part = 0.0f; since = 0; }
         }
     }
     two_sum_acc(hi, lo, part);
