@@ -361,7 +361,11 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float p) {
     hi = s;
 }
 
-constexpr int kDT = 16;      // per-line core table stride (entries 1..kNT32 used)
+// Per-line core table stride (entries 1..kNT32 used).  Odd, so that lanes
+// reading different lines' tables in one round of the core walk fall in
+// different LDS banks (a stride of 16 put every fourth line on one bank:
+// the c2 line-core pass 20 ms slower).
+constexpr int kDT = 17;
 // Largest x^2 + y^2 over a wave for the whole-wave 8-node FP32 path: keeps
 // D12 D34 ~ r2^8 below the FP32 range.  The whole-wave series path has no
 // such limit (its coefficients are bounded by y^2 < 4096).
@@ -433,28 +437,40 @@ __device__ __forceinline__ void eval32(const Rec32 &qr, float *part, int jr0) {
 // The records of one path code that this wave takes (mask m over records
 // c0 .. c0 + 63), in ascending order, with the next record's LDS fields
 // loaded while the current one is evaluated (the loads are the latency of a
-// record; an eval of the cheap paths is a few dozen instructions).
+// record; an eval of the cheap paths is a few dozen instructions).  Two
+// register sets take the records in turn, so no record is copied from one
+// set to the other (a copy per record was 6-8 VALU instructions, a fifth of
+// the far path's).
+template <int PPT>
+__device__ __forceinline__ void fold_parts(float *part, float *hi, float *lo, int &since) {
+    if (++since == 32) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
+        since = 0;
+    }
+}
+
 template <int PATH, bool FULL, int PPT>
 __device__ __forceinline__ void run_path(const Rec32 *rec, int c0, unsigned long long m,
                                          float *part, float *hi, float *lo, int &since,
                                          int jr0) {
     if (!m) return;
-    Rec32 cur = rec[c0 + lowest_bit(m)];
+    Rec32 a = rec[c0 + lowest_bit(m)];
     // the first record's loads complete here, so that inside the loop only
-    // the copy of the next record waits (after the current evaluation)
+    // the next record's loads are in flight during an evaluation
     __builtin_amdgcn_s_waitcnt(0xc07f);      // lgkmcnt(0)
     for (;;) {
-        const bool more = m != 0ull;
-        Rec32 nxt;
-        if (more) nxt = rec[c0 + lowest_bit(m)];
-        eval32<PATH, FULL, PPT>(cur, part, jr0);
-        if (++since == 32) {
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) { two_sum_acc(hi[k], lo[k], part[k]); part[k] = 0.0f; }
-            since = 0;
-        }
+        bool more = m != 0ull;
+        Rec32 b;
+        if (more) b = rec[c0 + lowest_bit(m)];
+        eval32<PATH, FULL, PPT>(a, part, jr0);
+        fold_parts<PPT>(part, hi, lo, since);
         if (!more) break;
-        cur = nxt;
+        more = m != 0ull;
+        if (more) a = rec[c0 + lowest_bit(m)];
+        eval32<PATH, FULL, PPT>(b, part, jr0);
+        fold_parts<PPT>(part, hi, lo, since);
+        if (!more) break;
     }
 }
 
