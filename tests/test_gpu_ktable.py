@@ -169,3 +169,31 @@ def test_sort_bins_mixed_lengths(E):
     Ks = E.sort_bins(_dev(K), seg).cpu().numpy()
     for b in range(len(lens)):
         np.testing.assert_array_equal(Ks[seg[b]:seg[b + 1]], np.sort(K[seg[b]:seg[b + 1]]))
+
+
+def test_sort_bins_signed_zeros(E):
+    """Bins holding negative zeros take the compare-and-select sort (min/max
+    instructions may return either of -0 and +0).  Sorted whole in LDS (no
+    segment of the call above 16384 values), every input bit pattern comes
+    out once and the values are in order; such bins sit beside ordinary
+    ones.  A call with a longer segment runs the bucket pass, which emits
+    the copies of a splitter value from the splitter: there the values are
+    compared."""
+    rng = np.random.default_rng(5)
+    bits = lambda a: np.sort(a.view(np.uint64))
+    lens = [10000, 16384, 257, 5000]
+    seg = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    K = np.round(rng.normal(0.0, 1.0, int(seg[-1])), 1)      # many +0 and -0
+    K[seg[2]:seg[3]] = np.abs(K[seg[2]:seg[3]]) + 1.0         # an ordinary bin
+    assert np.any(np.signbit(K[:seg[1]]) & (K[:seg[1]] == 0.0))
+    Ks = E.sort_bins(_dev(K), seg).cpu().numpy()
+    for b in range(len(lens)):
+        src, out = K[seg[b]:seg[b + 1]], Ks[seg[b]:seg[b + 1]]
+        np.testing.assert_array_equal(bits(out), bits(src))
+        np.testing.assert_array_equal(out, np.sort(src))
+    lens = [70000, 16384]
+    seg = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    K = np.round(rng.normal(0.0, 1.0, int(seg[-1])), 1)
+    Ks = E.sort_bins(_dev(K), seg).cpu().numpy()
+    for b in range(len(lens)):
+        np.testing.assert_array_equal(Ks[seg[b]:seg[b + 1]], np.sort(K[seg[b]:seg[b + 1]]))
