@@ -286,9 +286,11 @@ __global__ __launch_bounds__(kSortNT) void bin_sort_fit_kernel(const double *__r
         if (t < nC) fa.coeff[bid * nC + t] = bl(t);
         return;
     }
-    // 16 moments per pass over b keep the sums within the register budget
-    // of a 1024-thread workgroup
-    constexpr int G = kMomGroup / 2;
+    // 20 moments per pass over b (nC = 20, the usual order, in one pass)
+    // keep the sums within the register budget of a 1024-thread workgroup
+    // (32 per pass spill)
+    constexpr int G = 20;
+    static_assert(G <= kMomGroup, "block_sum rows");
     for (int g0 = 0; g0 < nC; g0 += G) {
         double m[G];
 #pragma unroll

@@ -168,7 +168,7 @@ __device__ __forceinline__ void lds_sort_run(double *s, int np, double (&v)[IPT]
 // output block comes from a bitonic half-cleaner; otherwise every step
 // compares and selects and the merge walks the runs one value at a time.
 template <int NT, int IPT>
-__device__ void lds_sort(double *s, int n) {
+__device__ __forceinline__ void lds_sort(double *s, int n) {
     static_assert(IPT == 16, "pad_index assumes 16 values per thread");
     const int t = threadIdx.x;
     const int np = (n + IPT - 1) / IPT * IPT;
