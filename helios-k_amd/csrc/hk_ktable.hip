@@ -57,6 +57,8 @@ constexpr int kBucketCap = kBucketNT * kIPT;   // values a bucket may hold (4096
 constexpr int kMaxRuns = 16;             // runs per bin the bucket pass takes
 constexpr int kMaxNC = 100;              // largest nC (the driver's def_NmaxSample)
 constexpr int kMomGroup = 32;            // moments accumulated per pass over a bin
+constexpr int kFitGroup = 20;            // the same in the bin and bucket fits (nC = 20
+                                         // in one pass within their register budgets)
 // The moment form c = P1 m + P2 b0 is exact algebra but inherits the
 // conditioning of the Chebyshev normal equations; for short bins
 // (Nxb < kPlanMinRatio * nC) the fit applies the Householder reflections
@@ -286,10 +288,9 @@ __global__ __launch_bounds__(kSortNT) void bin_sort_fit_kernel(const double *__r
         if (t < nC) fa.coeff[bid * nC + t] = bl(t);
         return;
     }
-    // 20 moments per pass over b (nC = 20, the usual order, in one pass)
-    // keep the sums within the register budget of a 1024-thread workgroup
-    // (32 per pass spill)
-    constexpr int G = 20;
+    // kFitGroup moments per pass over b keep the sums within the register
+    // budget of a 1024-thread workgroup (32 per pass spill)
+    constexpr int G = kFitGroup;
     static_assert(G <= kMomGroup, "block_sum rows");
     for (int g0 = 0; g0 < nC; g0 += G) {
         double m[G];
@@ -449,8 +450,10 @@ __global__ __launch_bounds__(kSortNT) void splitter_kernel(const double *__restr
 // run of values equal to s_lo is counted only.  Ranks o0 .. o1-1 of the
 // sorted segment are then: E copies of s_lo, then the sorted values; rank
 // o1 (the next bucket's first) is the smallest run value >= s_hi.
+// Four workgroups per CU: 128 VGPRs with a few spilled, faster than three
+// per CU without spills.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBucketNT) void bucket_kernel(const double *__restrict__ runs,
+__global__ __launch_bounds__(kBucketNT, 4) void bucket_kernel(const double *__restrict__ runs,
                                                            const Seg *__restrict__ segs,
                                                            int64_t nseg, SegMap map,
                                                            BucketArgs ba, FitArgs fa,
@@ -549,16 +552,16 @@ __global__ __launch_bounds__(kBucketNT) void bucket_kernel(const double *__restr
     __syncthreads();
     const int k0 = s_k[0], k1 = s_k[1];
     auto vali = [&](int r) { return val((int64_t)r); };
-    for (int g0 = 0; g0 < nC; g0 += kMomGroup) {
-        double m[kMomGroup];
+    for (int g0 = 0; g0 < nC; g0 += kFitGroup) {
+        double m[kFitGroup];
 #pragma unroll
-        for (int i = 0; i < kMomGroup; ++i) m[i] = 0.0;
+        for (int i = 0; i < kFitGroup; ++i) m[i] = 0.0;
         for (int k = k0 + t; k < k1; k += kBucketNT) {
             const double bv = rescaled_log(vali, k, nxm, Nxb, sres, fa.unit_scale);
             if (g0 == 0 && k < nC) fa.b0[bid * nC + k] = bv;
             add_moments(m, g0, nC, k * 2.0 / nlm1 - 1.0, bv);
         }
-        block_sum<kBucketNT / 64>(m, min(kMomGroup, nC - g0), red, mom + g0);
+        block_sum<kBucketNT / 64>(m, min(kFitGroup, nC - g0), red, mom + g0);
     }
     for (int i = t; i < nC; i += kBucketNT) part[i] = mom[i];
 }
