@@ -423,7 +423,10 @@ __global__ __launch_bounds__(kSortNT) void splitter_kernel(const double *__restr
         const int c = i / per_full, j = i % per_full;
         s[pad_index(i)] = base[(int64_t)c * R + (int64_t)(j + 1) * ba.m - 1];
     }
-    lds_sort<kSortNT, kIPT>(s, nsamp);
+    // each run's samples are in order: with a power-of-two count per run
+    // the sort starts from those runs
+    const bool pow2 = per_full >= kIPT && (per_full & (per_full - 1)) == 0;
+    lds_sort<kSortNT, kIPT>(s, nsamp, pow2 ? per_full : kIPT);
     const int nb = max(1, (nsamp + ba.g - 1) / ba.g);
     double *spl = ba.spl + f * (ba.bmax - 1);
     for (int w = 1 + t; w < nb; w += kSortNT) spl[w - 1] = s[pad_index(w * ba.g)];

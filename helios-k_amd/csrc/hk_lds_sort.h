@@ -103,14 +103,16 @@ __device__ __forceinline__ void load_window(const double *s, int i0, int m, doub
 
 template <int NT, int IPT, bool FAST>
 __device__ __forceinline__ void lds_sort_run(double *s, int np, double (&v)[IPT], bool act,
-                                             int d0) {
+                                             int d0, int L0) {
     if (act) {
-        if (FAST) sort_regs_minmax<IPT>(v);
-        else sort_regs<IPT>(v);
+        if (L0 <= IPT) {
+            if (FAST) sort_regs_minmax<IPT>(v);
+            else sort_regs<IPT>(v);
+        }
 #pragma unroll
         for (int q = 0; q < IPT; ++q) s[pad_index(d0 + q)] = v[q];
     }
-    for (int L = IPT; L < np; L <<= 1) {
+    for (int L = max(L0, IPT); L < np; L <<= 1) {
         __syncthreads();
         if (act) {
             const int ps = d0 / (2 * L) * (2 * L);
@@ -163,12 +165,13 @@ __device__ __forceinline__ void lds_sort_run(double *s, int np, double (&v)[IPT]
 
 // Sorts the n values at s[pad_index(i)], i < n <= NT*IPT, in place.  Every
 // thread of the workgroup (blockDim.x == NT) must call it; it begins and
-// ends with a barrier.  Without NaNs or negative zeros (checked here, once
+// ends with a barrier.  L0 (a power of two >= IPT): the values already form
+// sorted runs [r L0, (r + 1) L0), so the merge rounds start there.  Without NaNs or negative zeros (checked here, once
 // per call), compare-exchanges are min/max instructions and each merge
 // output block comes from a bitonic half-cleaner; otherwise every step
 // compares and selects and the merge walks the runs one value at a time.
 template <int NT, int IPT>
-__device__ __forceinline__ void lds_sort(double *s, int n) {
+__device__ __forceinline__ void lds_sort(double *s, int n, int L0 = IPT) {
     static_assert(IPT == 16, "pad_index assumes 16 values per thread");
     const int t = threadIdx.x;
     const int np = (n + IPT - 1) / IPT * IPT;
@@ -184,8 +187,8 @@ __device__ __forceinline__ void lds_sort(double *s, int n) {
             odd |= (v[q] != v[q]) | (v[q] == 0.0 && signbit(v[q]));
         }
     }
-    if (__syncthreads_or(odd)) lds_sort_run<NT, IPT, false>(s, np, v, act, d0);
-    else lds_sort_run<NT, IPT, true>(s, np, v, act, d0);
+    if (__syncthreads_or(odd)) lds_sort_run<NT, IPT, false>(s, np, v, act, d0, L0);
+    else lds_sort_run<NT, IPT, true>(s, np, v, act, d0, L0);
     __syncthreads();
 }
 
