@@ -891,6 +891,142 @@ __device__ __forceinline__ double cheb_eval(const double *c, double u) {
     return fma(u, b1, c[0] - b2);
 }
 
+// Window ends inside a wave (cut edges).  A far line whose cut window ends
+// inside the tile (about one cut from it) leaves, after the far field takes
+// the waves its window covers, a remainder in one wave: [lo, wave end) for a
+// line right of the tile (code 14), [wave start, hi) for one left of it
+// (code 15).  Such a line is smooth over the wave (band-1 distance: >= 16
+// wave half-widths), so each lane of a 64-record chunk expands its line over
+// the wave's interval in kEN Chebyshev terms (far32 at the nodes), and the
+// terms are summed over the lines whose windows reach each point: with the
+// lines' window ends in lane order, those lines are a prefix (right) or a
+// suffix (left) of the chunk, so an inclusive lane scan of the terms, read
+// at the point's count of open windows, gives the sum.  Chunks whose window
+// ends are not in lane order, or that hold both kinds, are taken record by
+// record on the far path.  kEN = 5: interpolation error < (1/32)^5 of a
+// line's value.
+constexpr int kEN = 5;
+__device__ __forceinline__ float edge_u(int k) {
+    return k == 0 ? 9.5105651630e-01f : k == 1 ? 5.8778525229e-01f : k == 2 ? 0.0f
+         : k == 3 ? -5.8778525229e-01f : -9.5105651630e-01f;
+}
+// (2 - [m == 0]) / kEN T_m(u_k)
+__device__ __forceinline__ float edge_w(int m, int k) {
+    constexpr float w[kEN][kEN] = {
+        {2.0e-01f, 2.0e-01f, 2.0e-01f, 2.0e-01f, 2.0e-01f},
+        {3.8042260652e-01f, 2.3511410092e-01f, 0.0f, -2.3511410092e-01f, -3.8042260652e-01f},
+        {3.2360679775e-01f, -1.2360679775e-01f, -4.0e-01f, -1.2360679775e-01f, 3.2360679775e-01f},
+        {2.3511410092e-01f, -3.8042260652e-01f, 0.0f, 3.8042260652e-01f, -2.3511410092e-01f},
+        {1.2360679775e-01f, -3.2360679775e-01f, 4.0e-01f, -3.2360679775e-01f, 1.2360679775e-01f}};
+    return w[m][k];
+}
+
+// Inclusive scans over the wave's lanes in lane order (DPP: rows of 16, then
+// rows 0+1 and 2+3, then all four); fixed order, so deterministic.  Every
+// lane must be active.
+__device__ __forceinline__ float wave_scan32(float v) {
+    v += dpp32<0x111, 0xf>(v);    // row_shr:1
+    v += dpp32<0x112, 0xf>(v);    // row_shr:2
+    v += dpp32<0x114, 0xf>(v);    // row_shr:4
+    v += dpp32<0x118, 0xf>(v);    // row_shr:8
+    v += dpp32<0x142, 0xa>(v);    // row_bcast:15
+    v += dpp32<0x143, 0xc>(v);    // row_bcast:31
+    return v;
+}
+
+__device__ __forceinline__ int wave_scan_max_i32(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+// ds_bpermute reads other lanes' values; a lane outside the exec mask reads
+// as 0, so these are called with every lane active (never under a branch).
+__device__ __forceinline__ float lane_get(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * l, __float_as_int(v)));
+}
+__device__ __forceinline__ int lane_get_i(int v, int l) {
+    return __builtin_amdgcn_ds_bpermute(4 * l, v);
+}
+
+// One chunk's edge records (nib 14 / 15 for this wave).  Returns false when
+// the chunk must be taken record by record.
+template <int PPT>
+__device__ __forceinline__ bool edge_chunk(const Rec32 *rec, int c0, int nst, int nib,
+                                           float *part, int jr0, float wcen, float whp) {
+    const int lane = threadIdx.x & 63;
+    const bool er = nib == 14, el = nib == 15;
+    const unsigned long long mr = __ballot(er), ml = __ballot(el);
+    if (mr && ml) return false;
+    const bool right = mr != 0ull;
+    const bool mine = right ? er : el;
+    // lane order of the scan: as loaded (right) or reversed (left), so that
+    // the lines reaching a point are a prefix
+    const int src = right ? lane : 63 - lane;
+    const Rec32 q = rec[min(c0 + src, nst - 1)];
+    const bool act = right ? ((mr >> src) & 1ull) : ((ml >> src) & 1ull);
+    (void)mine;
+    // keys: window start (right, non-decreasing) or end (left, taken as its
+    // negative so that the reversed order is non-decreasing too); lanes
+    // without a line take the running maximum
+    const int key = act ? (right ? q.lo : -q.hi) : INT_MIN;
+    const int run = wave_scan_max_i32(key);
+    if (__ballot(act && run != key)) return false;    // window ends out of order
+    // Chebyshev terms of this lane's line over the wave
+    float c[kEN];
+#pragma unroll
+    for (int m = 0; m < kEN; ++m) c[m] = 0.0f;
+    if (act) {
+        const float m0 = wcen + (float)q.d0;
+#pragma unroll
+        for (int k = 0; k < kEN; ++k) {
+            const float mm = fmaf(edge_u(k), whp, m0);
+            const float x = fmaf(mm, q.vbh, fmaf(mm, q.vbl, q.r));
+            const float v = far32(q, fmaf(x, x, q.y2));
+#pragma unroll
+            for (int m = 0; m < kEN; ++m) c[m] = fmaf(edge_w(m, k), v, c[m]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kEN; ++m) c[m] = wave_scan32(c[m]);
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int j = jr0 + 64 * k;
+        // lines reaching point j: key <= j (right: lo <= j; left: -hi <= -j - 1)
+        const int kj = right ? j : -j - 1;
+        int pos = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (lane_get_i(run, pos + step - 1) <= kj) pos += step;
+        // the halving steps reach at most 63: the 64th lane is one more test
+        const int r63 = lane_get_i(run, 63);
+        if (pos == 63 && r63 <= kj) pos = 64;
+        // the prefix sum at lane pos - 1 (none when pos == 0); the reads are
+        // unconditional, with every lane active
+        const int ls = max(pos - 1, 0);
+        float cs[kEN];
+#pragma unroll
+        for (int m = 0; m < kEN; ++m) {
+            const float t = lane_get(c[m], ls);
+            cs[m] = pos > 0 ? t : 0.0f;
+        }
+        const float u = ((float)j - wcen) / whp;
+        float b1 = 0.0f, b2 = 0.0f;
+#pragma unroll
+        for (int m = kEN - 1; m >= 1; --m) {
+            const float b0 = fmaf(2.0f * u, b1, cs[m] - b2);
+            b2 = b1;
+            b1 = b0;
+        }
+        part[k] += fmaf(u, b1, cs[0] - b2);
+    }
+    return true;
+}
+
 // Wave-level far field, node sums: lane l holds the record of its own line
 // (record c0 + l of the stage); the lanes whose line this wave takes in
 // band N (mine) add its values at the N nodes of the wave's interval
@@ -980,6 +1116,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
         // the line's window on the tile, [clo, chi) tile-relative, less the
         // waves the far field takes
         int32_t clo = 0, chi = 0;
+        bool edge1 = false;             // band-1 far line whose window ends in the tile
         // far lines first: FP32 values at the nodes, summed per wave (their
         // registers are free again before the direct records are built)
         {
@@ -998,6 +1135,7 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                     clo = (int32_t)(max(wl, j0) - j0);
                     chi = (int32_t)(min(wh, j0 + W) - j0);
                     if (band) {
+                        edge1 = band == 1;
                         cover = band == 1 && clo < chi ? far_cover(clo, chi, (int)Wv, 64 * PPT) : 0;
                         if (!cover) band = 0;
                     }
@@ -1073,6 +1211,17 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                     const int32_t wf = wave_far_codes<64 * PPT>(r.lo, r.hi, Wv, d0, vb, rr, y2);
                     r.code = (r.code & ~(((wf & 0x88888888) >> 3) * 0xF)) | wf;
                 }
+                if (edge1) {
+                    // the window's remainder in one wave, on the far path:
+                    // an edge record of that wave (14: right, 15: left)
+                    constexpr int wpts = 64 * PPT;
+                    const int ws = r.lo / wpts;
+                    if ((r.hi - 1) / wpts == ws && ((r.code >> (4 * ws)) & 15) == 2) {
+                        const int ea = ws * wpts, eb = (int)min((int64_t)(ea + wpts), Wv);
+                        const int ty = r.hi >= eb ? 14 : (r.lo <= ea ? 15 : 0);
+                        if (ty) r.code = (r.code & ~(15 << (4 * ws))) | (ty << (4 * ws));
+                    }
+                }
             }
         }
         // Barriers: the previous stage's readers of rec[] must be done before
@@ -1102,8 +1251,14 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
             run_path<kPathMid2, false, PPT>(rec, c0, __ballot(nib == 6), part, hi, lo, since, jr0);
             run_path<kPathAny, true, PPT>(rec, c0, __ballot(nib == 7), part, hi, lo, since, jr0);
             run_path<kPathAny, false, PPT>(rec, c0, __ballot(nib == 8), part, hi, lo, since, jr0);
+            if (__ballot(nib >= 14)) {
+                if (edge_chunk<PPT>(rec, c0, nst, nib, part, jr0, wcen, whp))
+                    fold_parts<PPT>(part, hi, lo, since);
+                else
+                    run_path<kPathFar, false, PPT>(rec, c0, __ballot(nib >= 14), part, hi, lo, since, jr0);
+            }
             // wave-level far field: each lane evaluates its own record's line
-            const unsigned long long wfm = __ballot(nib > 8);
+            const unsigned long long wfm = __ballot(nib > 8 && nib < 14);
             if (wfm) {
                 const Rec32 q = rec[min(c0 + lane, nst - 1)];
                 if (__ballot(nib == 9)) { wave_far_nodes<kFN1>(q, nib == 9, wcen, whp, ws, lane); wused |= 1; }
