@@ -349,7 +349,8 @@ def bench_single(args, w, wname):
         "voigt_regions_ABC": counts[1:4],
         "voigt_work": {"direct_pairs_ABC": work[0:3], "far_field_pairs": work[3],
                        "far_line_tiles_9_15_19_nodes": work[4:7],
-                       "far_line_waves_9_15_19_nodes": work[7:10]},
+                       "far_line_waves_9_15_19_nodes": work[7:10],
+                       "cut_edge_line_waves_and_pairs": work[10:12]},
         "voigt_kernel_ms": voigt_ms,
         "voigt_evals_per_s_kernel": evals / (voigt_ms * 1e-3),
         "fp32c_exact_pairs_voigt_ms": exact_ms,
@@ -483,7 +484,7 @@ def bench_batched(args, w, wname):
     run_lp()
     torch.cuda.synchronize()
     evals, regions = 0, [0, 0, 0]
-    work = [0] * 10
+    work = [0] * 12
     for st in states:
         c = E.pair_count(st, grid)
         evals += c[0]
@@ -607,7 +608,8 @@ def bench_batched(args, w, wname):
         "voigt_regions_ABC": regions,
         "voigt_work": {"direct_pairs_ABC": work[0:3], "far_field_pairs": work[3],
                        "far_line_tiles_9_15_19_nodes": work[4:7],
-                       "far_line_waves_9_15_19_nodes": work[7:10]},
+                       "far_line_waves_9_15_19_nodes": work[7:10],
+                       "cut_edge_line_waves_and_pairs": work[10:12]},
         "voigt_batched_ms": voigt_ms,
         "voigt_evals_per_s_kernel": evals / (voigt_ms * 1e-3),
         "fp32c_exact_pairs_batched_ms": exact_ms,

@@ -1682,7 +1682,11 @@ __global__ void zero4_kernel(unsigned long long *c) {
 // the far field, node evaluations) and direct pairs by region.  The band
 // decision is the kernel's (far_band on the same FP64 values).  out: direct
 // A, B, C pairs, far-field pairs, far (line, tile) counts of band 1 (9
-// nodes), band 2 (15 nodes) and band 3 (19 nodes).
+// nodes), band 2 (15 nodes) and band 3 (19 nodes), wave-level far (line,
+// wave) counts of bands 1-3, and the cut-edge remainders the kernel may take
+// through its per-wave expansion: (line, wave) count and their pairs (the
+// kernel takes them record by record in chunks whose window ends are out of
+// lane order or hold both kinds; counted here as expanded).
 __global__ __launch_bounds__(256) void work_count_kernel(const double *__restrict__ nu,
                                                          const double *__restrict__ hw,
                                                          const double *__restrict__ ia,
@@ -1690,7 +1694,7 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                                                          double numin, double dnu, int64_t nx,
                                                          int ppt, int far,
                                                          unsigned long long *__restrict__ out) {
-    constexpr int kOut = 10;
+    constexpr int kOut = 12;
     __shared__ unsigned long long red[kOut][256];
     const int t = threadIdx.x;
     const int W = kThreads * ppt, WP = 64 * ppt;
@@ -1714,6 +1718,7 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                 const double hW = 0.5 * (double)(Wv - 1);
                 bool full = false;
                 int band = far ? far_band(nu[i], hw[i], a, yy, j0, Wv, hW, numin, dnu, full) : 0;
+                const bool edge1 = band == 1 && !full;
                 // the pairs left to the pair-by-pair path: [dlo, dhi)
                 int32_t dlo = (int32_t)(max(wl, j0) - j0), dhi = (int32_t)(min(wh, j0 + Wv) - j0);
                 const int32_t n0 = dhi - dlo;
@@ -1725,11 +1730,13 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                 }
                 // waves that take the line from their own far field (the
                 // kernel's test, on the record's FP32 y)
-                int32_t wf = 0;
+                int32_t wf = 0, fr0 = 0, fr1 = 0;
                 if (far && Wv >= kFarMinPts && dlo < dhi) {
                     double rr, vb, d0;
                     line_offsets(nu[i], a, numin, dnu, j0, rr, vb, d0);
                     const double yf = (double)(float)yy, y2f = yf * yf;
+                    // points with x^2 + y^2 < kRF32 (the far path's bound)
+                    region_interval((double)kRF32, y2f, rr, vb, d0, W, fr0, fr1);
                     switch (ppt) {
                         case 2: wf = wave_far_codes<128>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
                         case 4: wf = wave_far_codes<256>(dlo, dhi, Wv, d0, vb, rr, y2f); break;
@@ -1745,6 +1752,16 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                         c[3] += (unsigned long long)(p1 - p0);
                         c[7 + wb - 9] += 1;
                         continue;
+                    }
+                    // a cut-edge remainder in one wave, all on the far path,
+                    // reaching the wave's end (right) or start (left)
+                    if (edge1 && (dhi - 1) / WP == dlo / WP && misses(fr0, fr1, w * WP, (w + 1) * WP)) {
+                        const int ea = w * WP, eb = (int)min((int64_t)(ea + WP), Wv);
+                        if (dhi >= eb || dlo <= ea) {
+                            c[10] += 1;
+                            c[11] += (unsigned long long)(p1 - p0);
+                            continue;
+                        }
                     }
                     const unsigned long long tot = ov(wl, wh, p0, p1);
                     const unsigned long long bc = ov(bl, bh, p0, p1);

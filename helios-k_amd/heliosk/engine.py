@@ -247,12 +247,13 @@ def profile(state, grid, profile, remove_plinth=False, kmin=0.0, voigt_mode=HK_V
 def work_count(state, grid, mode=HK_VOIGT_FP32C, tile_points=0, exact_pairs=False, stream=None):
     """(direct A, B, C pairs, far-field pairs, far (line, tile) pairs of the
     9-, 15- and 19-node bands, far (line, wave) pairs of the wave-level
-    bands) of one Voigt sum with this form and tiling."""
+    bands, cut-edge (line, wave) pairs and their point pairs) of one Voigt
+    sum with this form and tiling."""
     c = torch.zeros(12, dtype=torch.int64, device=_dev())
     tune = _tune(tile_points, exact_pairs)
     check(lib().hk_voigt_work_count(ctypes.byref(state.st), ctypes.byref(grid), int(mode),
                                     ctypes.byref(tune), _ptr(c), _stream(stream)))
-    return [int(v) for v in c.cpu().tolist()[:10]]
+    return [int(v) for v in c.cpu().tolist()]
 
 
 def pair_count(state, grid, stream=None):

@@ -227,10 +227,12 @@ int hk_voigt_pair_count(const hk_line_state *st, const hk_grid *grid,
  * through the FP32 far fields, far (line, tile) pairs of band 1 -- each 9
  * node evaluations --, of band 2 -- 15 each --, of band 3 -- 19 each --,
  * far (line, wave) pairs of the wave-level far field in bands 1, 2, 3 (the
- * same node counts), 0, 0].  A far line whose window ends inside a tile
- * counts as a far (line, tile) pair; its pairs in the wave that holds the
- * window's end are direct.  The direct and far-field pairs add up to
- * counts[0] of hk_voigt_pair_count. */
+ * same node counts), cut-edge (line, wave) pairs and their point pairs].  A
+ * far line whose window ends inside a tile counts as a far (line, tile)
+ * pair; its pairs in the wave that holds the window's end are cut-edge
+ * pairs when that wave takes them through its per-wave expansion (counted
+ * so whenever the line qualifies), direct otherwise.  The direct, far-field
+ * and cut-edge pairs add up to counts[0] of hk_voigt_pair_count. */
 int hk_voigt_work_count(const hk_line_state *st, const hk_grid *grid, int mode,
                         const hk_tune *tune, unsigned long long *counts, hk_stream_t stream);
 

@@ -357,10 +357,10 @@ def test_far_field_window_ends_inside_tile(E, tp, cut, n):
     e_ff, d_fe = rel_err(Kff, K64), rel_err(Kff, Kex)
     tot, A, B, C = E.pair_count(st, grid)
     w = E.work_count(st, grid, E.HK_VOIGT_FP32C, tile_points=tp)
-    print("tile %d: far field vs FP64 %.3e, vs every pair %.3e; direct %d far %d of %d"
-          % (tp, e_ff, d_fe, sum(w[:3]), w[3], tot))
+    print("tile %d: far field vs FP64 %.3e, vs every pair %.3e; direct %d far %d cut-edge %d of %d"
+          % (tp, e_ff, d_fe, sum(w[:3]), w[3], w[11], tot))
     assert e_ff <= 1e-5 and 0.0 < d_fe <= 2e-6
-    assert w[3] > 0 and sum(w[:4]) == tot
+    assert w[3] > 0 and sum(w[:4]) + w[11] == tot
     K = torch.empty(grid.nx, dtype=torch.float64, device="cuda")
     E.VoigtBatch([(st, grid, 0.0, K)], mode=E.HK_VOIGT_FP32C, tile_points=tp).launch()
     torch.cuda.synchronize()
@@ -368,16 +368,18 @@ def test_far_field_window_ends_inside_tile(E, tp, cut, n):
 
 
 def test_work_count_adds_up(E):
-    """Direct pairs by region plus far-field pairs equal the pair count; with
-    every pair evaluated (or in FP64) nothing goes through the far field."""
+    """Direct pairs by region plus far-field and cut-edge pairs equal the pair
+    count; with every pair evaluated (or in FP64) nothing goes through the
+    far field."""
     d = random_lines(20000, seed=46, numin=900.0, numax=1100.0)
     st = E.line_params(species_from(d), 1800.0, 2.0, Q=d["iso_Q"], cut_mode=0, cut=25.0)
     grid = E.make_grid(950.0, 1050.0, 0.001)
     tot, A, B, C = E.pair_count(st, grid)
     w = E.work_count(st, grid, E.HK_VOIGT_FP32C)
-    assert w[3] > 0 and all(v > 0 for v in w[4:7]) and sum(w[7:10]) > 0 and sum(w[:4]) == tot
+    assert w[3] > 0 and all(v > 0 for v in w[4:7]) and sum(w[7:10]) > 0
+    assert w[10] > 0 and w[11] > 0 and sum(w[:4]) + w[11] == tot
     for kw in (dict(mode=E.HK_VOIGT_FP32C, exact_pairs=True), dict(mode=E.HK_VOIGT_FP64)):
-        assert E.work_count(st, grid, **kw) == [A, B, C, 0, 0, 0, 0, 0, 0, 0]
+        assert E.work_count(st, grid, **kw) == [A, B, C] + [0] * 9
 
 
 @pytest.mark.parametrize("cm, cut", [(0, 25.0), (1, 400.0)])
