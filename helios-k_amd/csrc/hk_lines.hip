@@ -87,10 +87,13 @@ __global__ __launch_bounds__(256) void line_params_kernel(
     }
 }
 
-// out[0] = max over blocks (one workgroup, fixed order: deterministic).
+// out[r] = max over the n values v[r * n ..] (workgroup r; fixed order:
+// deterministic).
 __global__ __launch_bounds__(1024) void reduce_max_kernel(const double *__restrict__ v,
                                                           int64_t n, double *__restrict__ out) {
     __shared__ double s[1024];
+    v += (int64_t)blockIdx.x * n;
+    out += blockIdx.x;
     double m = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, v[i]);
     s[threadIdx.x] = m;
@@ -152,11 +155,9 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
     double cutv = cut->cut == 0.0 ? 1.0e30 : cut->cut;    // heliosk.cu:941
     hipStream_t s = (hipStream_t)stream;
     if (n == 0) {
-        for (int r = 0; r < 2; ++r) {
-            hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 2,
-                               (int64_t)0, st->radius + r);
-            HK_LAUNCH_CHECK();
-        }
+        hipLaunchKernelGGL(reduce_max_kernel, dim3(2), dim3(1024), 0, s, st->radius + 2,
+                           (int64_t)0, st->radius);
+        HK_LAUNCH_CHECK();
         return HK_OK;
     }
     IsoTab tab;
@@ -172,10 +173,9 @@ extern "C" int hk_line_params(const hk_lines *L, const hk_tp *tp, const hk_cut *
                        cut->cut_mode, cutv, st->nu, st->S, st->ialphaD, st->y, st->hw, st->ey,
                        st->blk_max, st->radius + 2, st->radius + 2 + nb);
     HK_LAUNCH_CHECK();
-    for (int r = 0; r < 2; ++r) {
-        hipLaunchKernelGGL(reduce_max_kernel, dim3(1), dim3(1024), 0, s, st->radius + 2 + r * nb,
-                           nb, st->radius + r);
-        HK_LAUNCH_CHECK();
-    }
+    // the two reaches (per-block values at radius + 2 and radius + 2 + nb)
+    hipLaunchKernelGGL(reduce_max_kernel, dim3(2), dim3(1024), 0, s, st->radius + 2, nb,
+                       st->radius);
+    HK_LAUNCH_CHECK();
     return HK_OK;
 }
