@@ -895,17 +895,22 @@ __device__ __forceinline__ double cheb_eval(const double *c, double u) {
 // inside the tile (about one cut from it) leaves, after the far field takes
 // the waves its window covers, a remainder in one wave: [lo, wave end) for a
 // line right of the tile (code 14), [wave start, hi) for one left of it
-// (code 15).  Such a line is smooth over the wave (band-1 distance: >= 16
-// wave half-widths), so each lane of a 64-record chunk expands its line over
+// (code 15).  Taken this way only when its centre lies >= kEdgeMin wave
+// half-widths from the wave's centre, where it is smooth over the wave: each
+// lane of a 64-record chunk expands its line over
 // the wave's interval in kEN Chebyshev terms (far32 at the nodes), and the
 // terms are summed over the lines whose windows reach each point: with the
 // lines' window ends in lane order, those lines are a prefix (right) or a
 // suffix (left) of the chunk, so an inclusive lane scan of the terms, read
 // at the point's count of open windows, gives the sum.  Chunks whose window
 // ends are not in lane order, or that hold both kinds, are taken record by
-// record on the far path.  kEN = 5: interpolation error < (1/32)^5 of a
-// line's value.
+// record on the far path.  kEN = 5 nodes keep the interpolation within
+// 3.8e-7 of a line's value at kEdgeMin = 16 half-widths (a window end
+// inside a band-1 tile can lie as near as 11, where 5 nodes would allow
+// 2.5e-6: those lines keep their records; at a cut of many tiles, as at
+// 100 cm^-1 and 1024 points, every cut edge is far beyond 16).
 constexpr int kEN = 5;
+constexpr double kEdgeMin = 16.0;
 __device__ __forceinline__ float edge_u(int k) {
     return k == 0 ? 9.5105651630e-01f : k == 1 ? 5.8778525229e-01f : k == 2 ? 0.0f
          : k == 3 ? -5.8778525229e-01f : -9.5105651630e-01f;
@@ -1212,11 +1217,14 @@ __device__ __forceinline__ void tile_f32c(const VArgs &a, int64_t tile, Rec32 *r
                     r.code = (r.code & ~(((wf & 0x88888888) >> 3) * 0xF)) | wf;
                 }
                 if (edge1) {
-                    // the window's remainder in one wave, on the far path:
-                    // an edge record of that wave (14: right, 15: left)
+                    // the window's remainder in one wave, on the far path,
+                    // from a centre >= kEdgeMin wave half-widths away: an
+                    // edge record of that wave (14: right, 15: left)
                     constexpr int wpts = 64 * PPT;
+                    constexpr double hwp = 0.5 * (wpts - 1);
                     const int ws = r.lo / wpts;
-                    if ((r.hi - 1) / wpts == ws && ((r.code >> (4 * ws)) & 15) == 2) {
+                    if ((r.hi - 1) / wpts == ws && ((r.code >> (4 * ws)) & 15) == 2 &&
+                        fabs((double)ws * wpts + hwp + d0) >= kEdgeMin * hwp) {
                         const int ea = ws * wpts, eb = (int)min((int64_t)(ea + wpts), Wv);
                         const int ty = r.hi >= eb ? 14 : (r.lo <= ea ? 15 : 0);
                         if (ty) r.code = (r.code & ~(15 << (4 * ws))) | (ty << (4 * ws));
@@ -1731,9 +1739,11 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                 // waves that take the line from their own far field (the
                 // kernel's test, on the record's FP32 y)
                 int32_t wf = 0, fr0 = 0, fr1 = 0;
+                double fd0 = 0.0;
                 if (far && Wv >= kFarMinPts && dlo < dhi) {
                     double rr, vb, d0;
                     line_offsets(nu[i], a, numin, dnu, j0, rr, vb, d0);
+                    fd0 = d0;
                     const double yf = (double)(float)yy, y2f = yf * yf;
                     // points with x^2 + y^2 < kRF32 (the far path's bound)
                     region_interval((double)kRF32, y2f, rr, vb, d0, W, fr0, fr1);
@@ -1755,7 +1765,8 @@ __global__ __launch_bounds__(256) void work_count_kernel(const double *__restric
                     }
                     // a cut-edge remainder in one wave, all on the far path,
                     // reaching the wave's end (right) or start (left)
-                    if (edge1 && (dhi - 1) / WP == dlo / WP && misses(fr0, fr1, w * WP, (w + 1) * WP)) {
+                    if (edge1 && (dhi - 1) / WP == dlo / WP && misses(fr0, fr1, w * WP, (w + 1) * WP) &&
+                        fabs((double)w * WP + 0.5 * (WP - 1) + fd0) >= kEdgeMin * 0.5 * (WP - 1)) {
                         const int ea = w * WP, eb = (int)min((int64_t)(ea + WP), Wv);
                         if (dhi >= eb || dlo <= ea) {
                             c[10] += 1;

@@ -49,3 +49,30 @@ def test_far_field_error_grows_toward_the_tile():
     nodes, edges = _constants()
     assert _interp_rel_err(lambda u: 1.0 / (u - edges[2]) ** 2, nodes[1]) > 1e-6
     assert _interp_rel_err(lambda u: 1.0 / (u - edges[1]) ** 2, nodes[0]) > 2e-7
+
+
+def test_cut_edge_expansion_error():
+    """Cut-edge remainders (a band-1 line whose window ends inside the tile)
+    are interpolated over one wave with the kernel's kEN nodes, for lines
+    whose centre lies >= kEdgeMin wave half-widths from the wave's centre;
+    nearer ones keep their records.  Band 1 alone would let such a line come
+    within 11 half-widths (the line lies beyond the tile's far end by the
+    band distance less a tile, and the wave's centre lies within a wave
+    half-width of the window's end), where kEN nodes would not hold: the
+    distance rule is what keeps the bound."""
+    src = open(os.path.join(ROOT, "helios-k_amd", "csrc", "hk_voigt.hip")).read()
+    n = int(re.search(r"constexpr int kEN = (\d+);", src).group(1))
+    amin = float(re.search(r"constexpr double kEdgeMin = ([\d.]+);", src).group(1))
+    _, edges = _constants()
+    W = 1024
+    hw = 0.5 * (W // 4 - 1)
+    a_band = ((W / 2 + edges[0] * 0.5 * (W - 1)) - (W - 1) - hw) / hw
+    assert 10.5 < a_band < 11.5
+    e = _interp_rel_err(lambda u: 1.0 / (u - amin) ** 2, n)
+    e_lor = _interp_rel_err(lambda u: 0.02 / ((u - amin) ** 2 + 0.02 ** 2), n)
+    print("cut edge n=%d a=%.1f: 1/(u-a)^2 %.3e, Lorentz %.3e" % (n, amin, e, e_lor))
+    assert e <= 4e-7 and e_lor <= 4e-7
+    # the rule matters: at band 1's own nearest distance, or with one node
+    # fewer, the bound fails
+    assert _interp_rel_err(lambda u: 1.0 / (u - a_band) ** 2, n) > 1e-6
+    assert _interp_rel_err(lambda u: 1.0 / (u - amin) ** 2, n - 1) > 4e-7
